@@ -1,0 +1,136 @@
+"""Benchmark: FP64 point-updates/s of the 2+1D Teukolsky evolution on one GPU.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+A step is one full SSP-RK3 time step (three fused stage kernels plus the
+extraction kernel) over the single-solve grid: N=16384 rho points on
+[-400, 1200] M, M=64 theta points, a=0.9, m=2, dt = 0.5 drho.  One
+point-update advances (Psi, Pi) at one grid point by one full step.
+
+Prints one JSON line with the throughput, the dominant kernel's roofline
+position (HIP events around every stage launch) and the host-core baseline
+(the CPU oracle's OpenMP build on a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+B_ALG_RK3 = 256.0              # algorithmic bytes per point-update, SSP-RK3
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_stage_pmc_16384x64.txt")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def pmc_traffic():
+    """HBM bytes per stage launch from the committed rocprofv3 --pmc summary."""
+    if not os.path.exists(PMC_SUMMARY):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(PMC_SUMMARY).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(p, seconds=12.0):
+    """Oracle (OpenMP build) on the first steps of the same workload."""
+    import numpy as np
+    from bh_ringdown_amd import teuk, workloads
+    from oracle import oracle as O
+    nth = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    _, r_c, th = teuk.grid(p)
+    op = O.make_params(p.N, p.M, p.xmin, p.xmax, p.a, p.m, dt=p.dt, j_extract=p.j_extract, nthreads=nth)
+    u = workloads.gaussian_pulse(p, seed=2, center=100.0, width=8.0)
+    lib = O.lib(timing=True)
+    coef = np.zeros((9, p.M, p.N))
+    lib.orc_coeffs(O.ctypes.byref(op), O._ptr(th), O._ptr(r_c), O._ptr(coef))
+    work = np.zeros((12, p.M, p.N))
+    step = lambda: lib.orc_step(O.ctypes.byref(op), O._ptr(coef), O._ptr(th), O._ptr(u), None, O._ptr(work))
+    t0 = time.perf_counter()
+    step()
+    per = time.perf_counter() - t0
+    n = int(max(3, min(2000, seconds / max(per, 1e-6))))
+    t0 = time.perf_counter()
+    for _ in range(n):
+        step()
+    el = time.perf_counter() - t0
+    return {"value": p.N * p.M * n / el, "unit": "point-updates/s", "cores": nth, "kind": "port",
+            "sample": f"{n} SSP-RK3 steps of the {p.N}x{p.M} grid (oracle, g++ -O3 OpenMP)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--timing-steps", type=int, default=300)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(independent solves shard one per GPU with no communication)", file=sys.stderr)
+        return 2
+
+    from bh_ringdown_amd import teuk, workloads
+    p = workloads.single_16384x64()
+    need = args.warmup + args.steps + args.timing_steps
+    p.T = max(p.T, need * p.dt * (1.0 + 1e-12))
+    _, r_c, th = teuk.grid(p)
+    u0 = workloads.gaussian_pulse(p, seed=2, center=100.0, width=8.0)
+    pts = p.N * p.M
+    log(f"bench: workload single_16384x64 N={p.N} M={p.M} warmup={args.warmup} steps={args.steps}")
+    with teuk.Solver(p, r_c, th) as s:
+        s.set_fields(u0)
+        s.step(args.warmup)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(args.steps)
+        s.sync()
+        el = time.perf_counter() - t0
+        log(f"bench: timed {args.steps} steps in {el:.3f} s")
+        # Kernel timing pass: HIP events around each stage launch, same stream.
+        s.enable_timing(True)
+        s.step(args.timing_steps)
+        s.sync()
+        launches, ms = s.timing()
+    value = pts * args.steps / el
+    per_launch_s = ms / 1e3 / launches
+    # a step is 3 stage launches; B_alg is per full step
+    achieved = (B_ALG_RK3 / 3.0) * pts / per_launch_s / 1e9
+    traffic = pmc_traffic()
+    out = {
+        "metric": "FP64 point-updates/s (%HBM roofline)",
+        "value": value,
+        "unit": "point-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "single_16384x64", "N": p.N, "M": p.M, "rho": [p.xmin, p.xmax],
+                   "a": p.a, "m": p.m, "dt_over_drho": 0.5, "integrator": "ssp-rk3",
+                   "source": "none (Gaussian initial data)"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_stage (fused RK stage)", "bytes_per_point_update": B_ALG_RK3,
+                     "avg_launch_us": per_launch_s * 1e6},
+    }
+    if not args.no_cpu:
+        log("bench: cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(p)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,590 @@
+// libteuk C ABI (include/teuk.h): solver lifecycle, stepping with
+// hipGraph-captured steps, trajectory consumption and the body() drop-in.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "teuk.h"
+#include "teuk_internal.h"
+
+using namespace teuk;
+
+struct teuk_solver {
+  teuk_params P;
+  Grid g;
+  DevState d;
+  hipStream_t stream = nullptr;
+  int nt = 0;              // scheduled steps: (int)(T/dt), body.cu:59
+  int k_host = 0;          // steps enqueued
+  bool has_src = false;
+  // Graph cache: one captured chunk of `graph_steps` steps.
+  hipGraphExec_t gexec = nullptr;
+  int graph_steps = 0;
+  // Kernel timing.
+  bool timing = false;
+  std::vector<hipEvent_t> ev;   // pairs
+  size_t ev_used = 0;
+  long long timed_launches = 0;
+  double timed_ms = 0.0;
+  std::vector<void *> allocs;
+};
+
+namespace {
+
+#define HIPCHK(x)                              \
+  do {                                         \
+    hipError_t e_ = (x);                       \
+    if (e_ != hipSuccess) return TEUK_EHIP;    \
+  } while (0)
+
+template <typename T>
+int dalloc(teuk_solver *s, T **p, size_t count) {
+  void *q = nullptr;
+  if (hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) return TEUK_ENOMEM;
+  s->allocs.push_back(q);
+  *p = static_cast<T *>(q);
+  return TEUK_OK;
+}
+
+int validate(const teuk_params *p) {
+  if (!p) return TEUK_EINVAL;
+  if (p->N < 128 || p->N % kWave != 0) return TEUK_EINVAL;
+  if (p->M < 16 || p->M > 256 || p->M % kRows != 0) return TEUK_EINVAL;
+  if (p->s != -2.0) return TEUK_EINVAL;  // kern.cu:142-145 bake in s = -2
+  if (!(p->dt > 0.0) || !(p->dx > 0.0) || !(p->dtheta > 0.0)) return TEUK_EINVAL;
+  if (!(p->xmax > p->xmin)) return TEUK_EINVAL;
+  if (p->Fr < 1 || p->ptsx < 1 || p->ptsy < 1) return TEUK_EINVAL;
+  if (p->integrator != 0 && p->integrator != 1) return TEUK_EINVAL;
+  if (p->j_extract >= p->N || p->j_extract < -1) return TEUK_EINVAL;
+  if (!(std::fabs(p->a) < p->mass)) return TEUK_EINVAL;  // horizon formulas need |a| < M
+  if (p->m != std::floor(p->m)) return TEUK_EINVAL;
+  if (!(p->T >= 0.0)) return TEUK_EINVAL;
+  return TEUK_OK;
+}
+
+int scheduled_steps(const teuk_params *p) { return (int)(p->T / p->dt); }
+
+void flush_timing(teuk_solver *s) {
+  if (s->ev_used == 0) return;
+  (void)hipStreamSynchronize(s->stream);
+  for (size_t i = 0; i < s->ev_used; i += 2) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, s->ev[i], s->ev[i + 1]) == hipSuccess) {
+      s->timed_ms += ms;
+      s->timed_launches += 1;
+    }
+  }
+  s->ev_used = 0;
+}
+
+int enqueue_step(teuk_solver *s) {
+  const int nstage = s->P.integrator == 0 ? 3 : 4;
+  if (s->has_src) HIPCHK(launch_source(s->g, s->d, s->stream));
+  for (int st = 1; st <= nstage; st++) {
+    if (s->timing) {
+      if (s->ev_used + 2 > s->ev.size()) {
+        if (s->ev.size() < 4096) {
+          for (int i = 0; i < 64; i++) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            s->ev.push_back(e);
+          }
+        } else {
+          flush_timing(s);
+        }
+      }
+      HIPCHK(hipEventRecord(s->ev[s->ev_used], s->stream));
+      HIPCHK(launch_stage(s->g, s->d, s->P.integrator, st, s->has_src, s->stream));
+      HIPCHK(hipEventRecord(s->ev[s->ev_used + 1], s->stream));
+      s->ev_used += 2;
+    } else {
+      HIPCHK(launch_stage(s->g, s->d, s->P.integrator, st, s->has_src, s->stream));
+    }
+  }
+  HIPCHK(launch_extract(s->g, s->d, s->stream));
+  return TEUK_OK;
+}
+
+int build_graph(teuk_solver *s, int steps) {
+  if (s->gexec) {
+    (void)hipGraphExecDestroy(s->gexec);
+    s->gexec = nullptr;
+  }
+  hipGraph_t graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  int rc = TEUK_OK;
+  for (int i = 0; i < steps && rc == TEUK_OK; i++) rc = enqueue_step(s);
+  hipError_t e = hipStreamEndCapture(s->stream, &graph);
+  if (rc != TEUK_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return TEUK_EHIP;
+  e = hipGraphInstantiate(&s->gexec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return TEUK_EHIP;
+  s->graph_steps = steps;
+  return TEUK_OK;
+}
+
+// body.cu:97-129 consumption of the .traj stream (10 columns per row):
+// read one row, then rows until t >= start; per step read two rows and use
+// the second; r += 1e-6; accelerations are backward differences of the
+// velocity columns over dt with zero velocities before the first step.
+int consume_trajectory(const double *rows, int nrows, int nsteps, double start, double dt,
+                       std::vector<SrcStep> &out) {
+  out.clear();
+  out.reserve(nsteps);
+  int cur = 0;
+  auto read = [&](const double *&row) {
+    if (cur >= nrows) return false;
+    row = rows + (size_t)10 * cur++;
+    return true;
+  };
+  const double *row = nullptr;
+  if (!read(row)) return TEUK_ERANGE;
+  do {
+    if (!read(row)) return TEUK_ERANGE;
+  } while (row[0] < start);
+  double v_r = 0.0, v_th = 0.0, v_ph = 0.0;
+  double timer = start;
+  for (int k = 0; k < nsteps; k++) {
+    if (!read(row) || !read(row)) return TEUK_ERANGE;
+    SrcStep st;
+    st.timer = timer;
+    st.start = start;
+    st.rp = row[1] + 1e-6;
+    st.tp = row[2];
+    st.drdt = row[3];
+    st.dthdt = row[4];
+    st.dphidt = row[5];
+    st.phip = row[6];
+    st.E = row[7];
+    st.lz = row[8];
+    st.Q = row[9];
+    st.d2rdt2 = (row[3] - v_r) / dt;
+    st.d2thdt2 = (row[4] - v_th) / dt;
+    st.d2phidt2 = (row[5] - v_ph) / dt;
+    v_r = row[3];
+    v_th = row[4];
+    v_ph = row[5];
+    out.push_back(st);
+    timer += dt;
+  }
+  return TEUK_OK;
+}
+
+int read_traj_file(const char *path, std::vector<double> &rows) {
+  FILE *f = std::fopen(path, "r");
+  if (!f) return TEUK_EIO;
+  double v[10];
+  while (std::fscanf(f, "%lf %lf %lf %lf %lf %lf %lf %lf %lf %lf", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5],
+                     &v[6], &v[7], &v[8], &v[9]) == 10)
+    rows.insert(rows.end(), v, v + 10);
+  std::fclose(f);
+  return rows.empty() ? TEUK_EIO : TEUK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *teuk_strerror(int code) {
+  switch (code) {
+    case TEUK_OK: return "ok";
+    case TEUK_EINVAL: return "invalid argument or unsupported size";
+    case TEUK_EHIP: return "HIP runtime error";
+    case TEUK_ENOMEM: return "out of memory";
+    case TEUK_EDIVERGED: return "non-finite field (solve diverged)";
+    case TEUK_EIO: return "trajectory or waveform file error";
+    case TEUK_ERANGE: return "step beyond the scheduled run or trajectory";
+    default: return "unknown error";
+  }
+}
+
+int teuk_default_params(teuk_params *p, int N, int M, double xmin, double xmax, double a, double m,
+                        double dt_over_dx, double T) {
+  if (!p || N <= 0 || M <= 0) return TEUK_EINVAL;
+  std::memset(p, 0, sizeof(*p));
+  p->N = N;
+  p->M = M;
+  p->xmin = xmin;
+  p->xmax = xmax;
+  p->mass = 1.0;
+  p->a = a;
+  p->s = -2.0;
+  p->m = m;
+  p->dx = (xmax - xmin) / N;
+  p->dtheta = kPi15 / M;
+  p->dt = dt_over_dx * p->dx;
+  p->T = T;
+  p->Fr = 1;
+  p->ptsx = 128;
+  p->ptsy = 128;
+  p->integrator = 0;
+  p->j_extract = -1;
+  return TEUK_OK;
+}
+
+int teuk_grid(const teuk_params *p, double *x_c, double *r_c, double *theta) {
+  if (!p || !x_c || !r_c || !theta || p->N <= 0 || p->M <= 0) return TEUK_EINVAL;
+  host_grid(p->N, p->M, p->xmin, p->xmax, p->mass, p->a, x_c, r_c, theta);
+  return TEUK_OK;
+}
+
+long long teuk_footprint(const teuk_params *p, int nsteps) {
+  if (!p) return -1;
+  const long long NM = (long long)p->N * p->M;
+  const long long states = (p->integrator == 0 ? 3 : 4) * 4 * NM;
+  const long long nrec = (long long)(std::max(nsteps, 0) / std::max(p->Fr, 1)) + 1;
+  long long bytes = 8 * (states + 9 * NM + p->N + 3 * (long long)p->M);
+  bytes += 8 * (2LL * p->ptsx * p->ptsy) + 8;
+  bytes += 8 * (4LL * p->M + 2 * p->M + 1 + 7LL * 5 * (p->M / 4) * 3);
+  bytes += nrec * (16 * 8 + 2 * 4 + 2LL * p->M * 8);
+  bytes += (long long)nsteps * (long long)sizeof(SrcStep);
+  return bytes;
+}
+
+int teuk_create(const teuk_params *p, const double *r_c, const double *theta, teuk_solver **out) {
+  if (!out || !r_c || !theta) return TEUK_EINVAL;
+  *out = nullptr;
+  int rc = validate(p);
+  if (rc) return rc;
+  teuk_solver *s = new teuk_solver();
+  s->P = *p;
+  s->nt = scheduled_steps(p);
+  Grid &g = s->g;
+  g.N = p->N;
+  g.M = p->M;
+  g.even_m = (((int)p->m) % 2) == 0;
+  g.ptsx = p->ptsx;
+  g.ptsy = p->ptsy;
+  g.je = p->j_extract < 0 ? p->N - 1 : p->j_extract;
+  g.Fr = p->Fr;
+  g.nrec_cap = s->nt / p->Fr + 1;
+  g.xmin = p->xmin;
+  g.xmax = p->xmax;
+  g.dx = p->dx;
+  g.dtheta = p->dtheta;
+  g.dt = p->dt;
+  g.mass = p->mass;
+  g.a = p->a;
+  g.m = p->m;
+  g.start = 0.0;
+
+  auto fail = [&](int code) {
+    teuk_destroy(s);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return fail(TEUK_EHIP);
+  const size_t NM = (size_t)p->N * p->M;
+  DevState &d = s->d;
+  const int nstates = p->integrator == 0 ? 3 : 4;
+  double *states = nullptr;
+  if ((rc = dalloc(s, &states, nstates * 4 * NM))) return fail(rc);
+  d.U = states;
+  d.C = states + 4 * NM;
+  d.D = states + 8 * NM;
+  d.A = nstates == 4 ? states + 12 * NM : nullptr;
+  if ((rc = dalloc(s, &d.coef, 9 * NM))) return fail(rc);
+  if ((rc = dalloc(s, &d.ctan, p->M))) return fail(rc);
+  if ((rc = dalloc(s, &d.theta, p->M))) return fail(rc);
+  if ((rc = dalloc(s, &d.r_c, p->N))) return fail(rc);
+  if ((rc = dalloc(s, &d.patch, 2 * (size_t)p->ptsx * p->ptsy))) return fail(rc);
+  if ((rc = dalloc(s, &d.patch_org, 2))) return fail(rc);
+  if ((rc = dalloc(s, &d.ext, 4 * (size_t)p->M))) return fail(rc);
+  if ((rc = dalloc(s, &d.extw, 2 * (size_t)p->M + 1))) return fail(rc);
+  const size_t nproj = 7 * (size_t)(5 * (p->M / 4)) * 3;
+  if ((rc = dalloc(s, &d.projw, nproj))) return fail(rc);
+  if ((rc = dalloc(s, &d.rec, (size_t)g.nrec_cap * 16))) return fail(rc);
+  if ((rc = dalloc(s, &d.rec_k, (size_t)g.nrec_cap * 2))) return fail(rc);
+  if ((rc = dalloc(s, &d.psi, (size_t)g.nrec_cap * 2 * p->M))) return fail(rc);
+  if ((rc = dalloc(s, &d.ctl, 1))) return fail(rc);
+  d.steps = nullptr;
+  d.nsteps_src = 0;
+
+  // Zero fields and integrals (kern.cu:164-167, body.cu:86-91).
+  if (hipMemsetAsync(states, 0, nstates * 4 * NM * sizeof(double), s->stream) != hipSuccess) return fail(TEUK_EHIP);
+  if (hipMemsetAsync(d.ext, 0, 4 * (size_t)p->M * sizeof(double), s->stream) != hipSuccess) return fail(TEUK_EHIP);
+  if (hipMemsetAsync(d.patch, 0, 2 * (size_t)p->ptsx * p->ptsy * sizeof(double), s->stream) != hipSuccess)
+    return fail(TEUK_EHIP);
+  Ctl c0{};
+  c0.k = 0;
+  c0.nrec = 0;
+  c0.status = 0;
+  c0.timer = g.start;
+  if (hipMemcpyAsync(d.ctl, &c0, sizeof(Ctl), hipMemcpyHostToDevice, s->stream) != hipSuccess)
+    return fail(TEUK_EHIP);
+  if (hipMemcpyAsync(d.theta, theta, p->M * sizeof(double), hipMemcpyHostToDevice, s->stream) != hipSuccess)
+    return fail(TEUK_EHIP);
+  if (hipMemcpyAsync(d.r_c, r_c, p->N * sizeof(double), hipMemcpyHostToDevice, s->stream) != hipSuccess)
+    return fail(TEUK_EHIP);
+  int org[2] = {-(1 << 29), -(1 << 29)};
+  if (hipMemcpyAsync(d.patch_org, org, sizeof(org), hipMemcpyHostToDevice, s->stream) != hipSuccess)
+    return fail(TEUK_EHIP);
+
+  // Extraction weights, body.cu:175-183 (rr is the extraction radius).
+  std::vector<double> extw(2 * p->M + 1);
+  const double rr = r_c[g.je];
+  for (int q = 0; q < p->M; q++) {
+    const double rho2 = rr * rr + std::pow(p->a * std::cos(theta[q]), 2);
+    extw[q] = 0.5 * p->dtheta * std::sin(theta[q]) * rr * rr;
+    extw[p->M + q] = std::pow(rho2, 4);
+  }
+  extw[2 * p->M] = std::pow(rr, 6);
+  std::vector<double> projw(nproj);
+  host_projection_weights(p->M, p->m, projw.data());
+  if (hipMemcpyAsync(d.extw, extw.data(), extw.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) !=
+      hipSuccess)
+    return fail(TEUK_EHIP);
+  if (hipMemcpyAsync(d.projw, projw.data(), projw.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) !=
+      hipSuccess)
+    return fail(TEUK_EHIP);
+  if (launch_init_coeffs(g, d, s->stream) != hipSuccess) return fail(TEUK_EHIP);
+  if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(TEUK_EHIP);
+  *out = s;
+  return TEUK_OK;
+}
+
+void teuk_destroy(teuk_solver *s) {
+  if (!s) return;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
+  for (void *q : s->allocs) (void)hipFree(q);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+int teuk_set_fields(teuk_solver *s, const double *u) {
+  if (!s || !u) return TEUK_EINVAL;
+  const size_t bytes = 4 * (size_t)s->P.N * s->P.M * sizeof(double);
+  HIPCHK(hipMemcpyAsync(s->d.U, u, bytes, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return TEUK_OK;
+}
+
+int teuk_get_fields(teuk_solver *s, double *u) {
+  if (!s || !u) return TEUK_EINVAL;
+  const size_t bytes = 4 * (size_t)s->P.N * s->P.M * sizeof(double);
+  HIPCHK(hipMemcpyAsync(u, s->d.U, bytes, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return TEUK_OK;
+}
+
+int teuk_set_source(teuk_solver *s, const teuk_source_step *steps, int nsteps) {
+  if (!s || nsteps < 0) return TEUK_EINVAL;
+  if (!steps || nsteps == 0) {
+    s->has_src = false;
+    return TEUK_OK;
+  }
+  if (!source_available()) return TEUK_EINVAL;
+  static_assert(sizeof(teuk_source_step) == sizeof(SrcStep), "source step layout");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  SrcStep *dst = nullptr;
+  int rc = dalloc(s, &dst, (size_t)nsteps);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(dst, steps, (size_t)nsteps * sizeof(SrcStep), hipMemcpyHostToDevice));
+  s->d.steps = dst;
+  s->d.nsteps_src = nsteps;
+  s->has_src = true;
+  if (s->gexec) {  // captured graphs hold the old pointers
+    (void)hipGraphExecDestroy(s->gexec);
+    s->gexec = nullptr;
+  }
+  return TEUK_OK;
+}
+
+int teuk_set_trajectory(teuk_solver *s, const double *rows, int nrows, int nsteps) {
+  if (!s || !rows || nrows <= 0 || nsteps <= 0) return TEUK_EINVAL;
+  std::vector<SrcStep> sched;
+  int rc = consume_trajectory(rows, nrows, nsteps, s->g.start, s->P.dt, sched);
+  if (rc) return rc;
+  return teuk_set_source(s, reinterpret_cast<const teuk_source_step *>(sched.data()), nsteps);
+}
+
+int teuk_trajectory_schedule(const double *rows, int nrows, int nsteps, double start, double dt,
+                             teuk_source_step *out) {
+  if (!rows || !out || nrows <= 0 || nsteps <= 0 || !(dt > 0.0)) return TEUK_EINVAL;
+  std::vector<SrcStep> sched;
+  int rc = consume_trajectory(rows, nrows, nsteps, start, dt, sched);
+  if (rc) return rc;
+  std::memcpy(out, sched.data(), sched.size() * sizeof(SrcStep));
+  return TEUK_OK;
+}
+
+int teuk_step(teuk_solver *s, int nsteps) {
+  if (!s || nsteps < 0) return TEUK_EINVAL;
+  if (s->k_host + nsteps > s->nt) return TEUK_ERANGE;
+  if (s->has_src && s->k_host + nsteps > s->d.nsteps_src) return TEUK_ERANGE;
+  int rc = TEUK_OK;
+  if (s->timing) {
+    for (int i = 0; i < nsteps && rc == TEUK_OK; i++) rc = enqueue_step(s);
+  } else {
+    const int G = 32;
+    int left = nsteps;
+    if (left >= G) {
+      if (!s->gexec || s->graph_steps != G) rc = build_graph(s, G);
+      while (rc == TEUK_OK && left >= G) {
+        if (hipGraphLaunch(s->gexec, s->stream) != hipSuccess) rc = TEUK_EHIP;
+        left -= G;
+      }
+    }
+    for (int i = 0; i < left && rc == TEUK_OK; i++) rc = enqueue_step(s);
+  }
+  if (rc == TEUK_OK) s->k_host += nsteps;
+  return rc;
+}
+
+int teuk_sync(teuk_solver *s) {
+  if (!s) return TEUK_EINVAL;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (s->timing) flush_timing(s);
+  Ctl c;
+  HIPCHK(hipMemcpy(&c, s->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+  return c.status ? TEUK_EDIVERGED : TEUK_OK;
+}
+
+int teuk_progress(teuk_solver *s, int *k, double *t) {
+  if (!s) return TEUK_EINVAL;
+  Ctl c;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipMemcpy(&c, s->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+  if (k) *k = c.k;
+  if (t) *t = c.timer;
+  return TEUK_OK;
+}
+
+int teuk_read_output(teuk_solver *s, int first, int max, teuk_output_row *rows, int *n) {
+  if (!s || !rows || first < 0 || max < 0) return TEUK_EINVAL;
+  Ctl c;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipMemcpy(&c, s->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+  const int cnt = std::max(0, std::min(max, c.nrec - first));
+  if (cnt > 0) {
+    std::vector<double> rec((size_t)cnt * 16);
+    std::vector<int> rk((size_t)cnt * 2);
+    HIPCHK(hipMemcpy(rec.data(), s->d.rec + (size_t)first * 16, rec.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(rk.data(), s->d.rec_k + (size_t)first * 2, rk.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (int i = 0; i < cnt; i++) {
+      rows[i].k = rk[2 * i];
+      rows[i].write = rk[2 * i + 1];
+      rows[i].t = rec[16 * i];
+      rows[i].dedt = rec[16 * i + 1];
+      for (int j = 0; j < 14; j++) rows[i].h[j] = rec[16 * i + 2 + j];
+    }
+  }
+  if (n) *n = cnt;
+  return TEUK_OK;
+}
+
+int teuk_read_psi(teuk_solver *s, int first, int max, double *psi, int *n) {
+  if (!s || !psi || first < 0 || max < 0) return TEUK_EINVAL;
+  Ctl c;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipMemcpy(&c, s->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+  const int cnt = std::max(0, std::min(max, c.nrec - first));
+  if (cnt > 0)
+    HIPCHK(hipMemcpy(psi, s->d.psi + (size_t)first * 2 * s->P.M, (size_t)cnt * 2 * s->P.M * sizeof(double),
+                     hipMemcpyDeviceToHost));
+  if (n) *n = cnt;
+  return TEUK_OK;
+}
+
+int teuk_enable_timing(teuk_solver *s, int on) {
+  if (!s) return TEUK_EINVAL;
+  if (s->timing && !on) flush_timing(s);
+  s->timing = on != 0;
+  s->timed_ms = 0.0;
+  s->timed_launches = 0;
+  return TEUK_OK;
+}
+
+int teuk_timing(teuk_solver *s, long long *launches, double *ms) {
+  if (!s) return TEUK_EINVAL;
+  flush_timing(s);
+  if (launches) *launches = s->timed_launches;
+  if (ms) *ms = s->timed_ms;
+  return TEUK_OK;
+}
+
+int teuk_body(const teuk_params *p, const double *r_c, const double *theta, const char *traj_path,
+              const char *out_path, const char *full_path, int verbose) {
+  int rc = validate(p);
+  if (rc) return rc;
+  teuk_solver *s = nullptr;
+  if ((rc = teuk_create(p, r_c, theta, &s))) return rc;
+  const int nt = s->nt;
+  if (traj_path) {
+    std::vector<double> rows;
+    if ((rc = read_traj_file(traj_path, rows))) {
+      teuk_destroy(s);
+      return rc;
+    }
+    if ((rc = teuk_set_trajectory(s, rows.data(), (int)(rows.size() / 10), nt))) {
+      teuk_destroy(s);
+      return rc;
+    }
+  }
+  FILE *fo = out_path ? std::fopen(out_path, "w") : nullptr;
+  FILE *ff = full_path ? std::fopen(full_path, "w") : nullptr;
+  if ((out_path && !fo) || (full_path && !ff)) {
+    if (fo) std::fclose(fo);
+    if (ff) std::fclose(ff);
+    teuk_destroy(s);
+    return TEUK_EIO;
+  }
+  if (verbose) {
+    std::printf(" procs %d my_rank %d \n", 1, 0);
+    std::printf(" start %lf endtime %lf \n", 0.0, p->T);
+  }
+  // Advance in slices so output rows stream out while the GPU runs.
+  const int slice = std::max(p->Fr, 1) * 64;
+  int done = 0, nread = 0;
+  std::vector<teuk_output_row> rows(1024);
+  while (rc == TEUK_OK && done < nt) {
+    const int n = std::min(slice, nt - done);
+    rc = teuk_step(s, n);
+    if (rc) break;
+    done += n;
+    rc = teuk_sync(s);
+    int got = 0;
+    do {
+      int r2 = teuk_read_output(s, nread, (int)rows.size(), rows.data(), &got);
+      if (r2) {
+        rc = r2;
+        break;
+      }
+      for (int i = 0; i < got; i++) {
+        const teuk_output_row &o = rows[i];
+        if (o.write) {
+          // body.cu:208 (time column is timer - Xmax)
+          if (fo) {
+            std::fprintf(fo, "%lf", o.t - p->xmax);
+            for (int j = 0; j < 14; j++) std::fprintf(fo, " %g", o.h[j]);
+            std::fprintf(fo, "\n");
+          }
+          if (ff) {
+            std::fprintf(ff, "%.17g", o.t - p->xmax);
+            for (int j = 0; j < 14; j++) std::fprintf(ff, " %.17g", o.h[j]);
+            std::fprintf(ff, "\n");
+          }
+        }
+        if (verbose) std::printf("%lf %g \n", o.t, o.dedt);  // body.cu:210
+      }
+      nread += got;
+    } while (got == (int)rows.size());
+  }
+  if (fo) std::fclose(fo);
+  if (ff) std::fclose(ff);
+  teuk_destroy(s);
+  return rc;
+}
+
+}  // extern "C"

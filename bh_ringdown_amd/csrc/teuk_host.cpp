@@ -1,0 +1,114 @@
+// Host-side set-up for libteuk: the compactified grid with its Newton
+// inversion r(r*), the -2Y_lm projection weights and the trajectory
+// consumption pattern of the reference driver.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "teuk_internal.h"
+
+namespace teuk {
+
+// TD_GPU_code/HighOrder/main.c:79-198.  rho_l = Xmin + l dx, theta_k = k dtheta,
+// Omega(rho) = 1 inside rin = Xmin + 0.64 (Xmax - Xmin) and
+// 1 - ((rho - rin)/(Xmax + 1e-6 - rin))^4 beyond, r* = rho/Omega, then
+// Newton on r*(r) - r* = 0 with dr*/dr = (r^2+a^2)/Delta, marching inward
+// from the outer edge with each point warm-started from its neighbour.
+void host_grid(int N, int M, double xmin, double xmax, double mass, double a, double *x_c, double *r_c,
+               double *theta) {
+  const double dx = (xmax - xmin) / N;
+  const double dth = kPi15 / M;
+  for (int l = 0; l < N; l++) x_c[l] = xmin + (double)l * dx;
+  for (int k = 0; k < M; k++) theta[k] = (double)k * dth;
+  const double rin = xmin + (xmax - xmin) * 0.64;
+  const double outer = xmax + 1e-6;
+  std::vector<double> target(N);
+  for (int l = 0; l < N; l++) {
+    const double om = (x_c[l] < rin) ? 1.0 : 1.0 - std::pow((x_c[l] - rin) / (outer - rin), 4);
+    target[l] = x_c[l] / om;
+  }
+  if (mass == 0.0) {
+    for (int l = 0; l < N; l++) r_c[l] = target[l];
+    return;
+  }
+  const double disc = std::sqrt(mass * mass - a * a);
+  const double rp = mass + disc, rm = mass - disc;
+  const double tol = 1e-12;
+  double r = target[N - 1];
+  for (int l = N - 1; l >= 0; l--) {
+    double step_rel, r_new;
+    do {
+      const double jac = (r * r + a * a - 2.0 * mass * r) / (r * r + a * a);
+      double rstar;
+      if (mass == a)
+        rstar = r + 2.0 * a * std::log(r / a - 1.0) - 2.0 * a * a / (r - a);
+      else if (a == 0.0)
+        rstar = r + 2.0 * mass * std::log(r / (2.0 * mass) - 1.0);
+      else
+        rstar = r + (rp * rp + a * a) / (rp - rm) * std::log(r / rp - 1.0) -
+                (rm * rm + a * a) / (rp - rm) * std::log(r / rm - 1.0);
+      r_new = r + (target[l] - rstar) * jac;
+      // Past double resolution of r - r+ the reference iteration hits
+      // log(0) and returns NaN for every inner point; clamp to r+ (the
+      // r* -> -inf limit) there.  Finite reference results are unchanged.
+      if (!(r_new > rp)) {
+        r_new = rp;
+        r = r_new;
+        break;
+      }
+      step_rel = std::fabs(r_new - r) / r;
+      r = r_new;
+    } while (step_rel > tol);
+    r_c[l] = r_new;
+  }
+}
+
+static double factorial(double n) {
+  double f = 1.0;
+  for (double c = 1.0; c <= n; c++) f = f * c;
+  return f;
+}
+
+// proj.c:16-35: -2Y_lm(theta) (phi part dropped) from the Wigner d-matrix sum.
+double host_swsh(double ell, double ml, double the) {
+  const double tlo = (ml - 2.0) > 0 ? ml - 2.0 : 0.0;
+  const double thi = (ell - 2.0) > (ell + ml) ? ml + ell : ell - 2.0;
+  double acc = 0.0;
+  for (double t = tlo; t <= thi; t++) {
+    double c = std::pow(-1.0, t) *
+               std::sqrt(factorial(ell + ml) * factorial(ell - ml) * factorial(ell + 2.0) * factorial(ell - 2.0));
+    c = c / (factorial(ell + ml - t) * factorial(ell - 2.0 - t) * factorial(t) * factorial(t + 2.0 - ml));
+    acc = acc + c * std::pow(std::cos(the / 2.0), 2.0 * ell + ml - 2.0 - 2.0 * t) *
+                    std::pow(std::sin(the / 2.0), 2.0 * t + 2.0 - ml);
+  }
+  return std::sqrt((2.0 * ell + 1.0) / 4.0 / kPi15) * acc;
+}
+
+// Factors of the composite Boole rule of proj.c:47-72 for l = |m| .. |m|+6.
+// The reference sum has 5 terms per panel (panel joints appear twice, at
+// slightly different accumulated theta), so the table holds one entry per
+// term t = 5 i + j, evaluated at grid point 4 i + j (point M repeats M-1,
+// proj.c:45): (w_j*4*Pi, sin(theta_t), -2Y_lm(theta_t)), with theta_t
+// accumulated exactly as the reference does (4 dtheta i, then += dtheta).
+void host_projection_weights(int M, double m, double *w) {
+  const double dth = kPi15 / M;
+  static const double boole[5] = {7.0, 32.0, 12.0, 32.0, 7.0};
+  const int am = std::abs((int)m);
+  const int nterm = 5 * (M / 4);
+  for (int l = 0; l < 7; l++) {
+    double *wl = w + (size_t)l * nterm * 3;
+    for (int i = 0; i < M / 4; i++) {
+      double th = 4.0 * dth * (double)i;
+      for (int j = 0; j < 5; j++) {
+        if (j > 0) th += dth;
+        double *e = wl + 3 * (5 * i + j);
+        e[0] = boole[j] * 4.0 * kPi15;
+        e[1] = std::sin(th);
+        e[2] = host_swsh((double)(l + am), m, th);
+      }
+    }
+  }
+}
+
+}  // namespace teuk

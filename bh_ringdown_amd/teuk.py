@@ -1,0 +1,266 @@
+"""Python view of libteuk's C ABI (include/teuk.h), for tests and benchmarks.
+
+Mirrors the reference driver's interface (TD_GPU_code/HighOrder):
+``grid`` is main.c:79-198 grid(), ``body`` is body.cu:25-225 body(), and
+``Solver`` exposes the per-step operations body() is built from.  All
+arithmetic runs in libteuk (HIP kernels on the GPU plus its host set-up);
+this module only marshals arrays.  If the library has not been built, every
+entry point raises -- there is no fallback path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libteuk.so")
+PI15 = 3.14159265358979  # parm.h:12
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+TEUK_OK, TEUK_EINVAL, TEUK_EHIP, TEUK_ENOMEM, TEUK_EDIVERGED, TEUK_EIO, TEUK_ERANGE = 0, -1, -2, -3, -4, -5, -6
+
+
+class TeukError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__(f"{what}: {strerror(code)} ({code})")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    """teuk_params (include/teuk.h); names follow parm.h:1-11 / main.c:55-61."""
+    _fields_ = [
+        ("N", ctypes.c_int), ("M", ctypes.c_int),
+        ("xmin", ctypes.c_double), ("xmax", ctypes.c_double),
+        ("mass", ctypes.c_double), ("a", ctypes.c_double),
+        ("s", ctypes.c_double), ("m", ctypes.c_double),
+        ("dx", ctypes.c_double), ("dtheta", ctypes.c_double), ("dt", ctypes.c_double),
+        ("T", ctypes.c_double),
+        ("Fr", ctypes.c_int), ("ptsx", ctypes.c_int), ("ptsy", ctypes.c_int),
+        ("integrator", ctypes.c_int), ("j_extract", ctypes.c_int),
+    ]
+
+    def nsteps(self):
+        return int(self.T / self.dt)
+
+    def __repr__(self):
+        return "Params(" + ", ".join(f"{n}={getattr(self, n)!r}" for n, _ in self._fields_) + ")"
+
+
+class SourceStep(ctypes.Structure):
+    """teuk_source_step: the per-step scalars body.cu:122-124 hands to sourced."""
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "timer", "start", "rp", "tp", "phip", "E", "lz", "Q",
+        "drdt", "d2rdt2", "dthdt", "d2thdt2", "dphidt", "d2phidt2")]
+
+
+class OutputRow(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_int), ("write", ctypes.c_int), ("t", ctypes.c_double),
+                ("dedt", ctypes.c_double), ("h", ctypes.c_double * 14)]
+
+
+EXPORTS = [
+    "teuk_grid", "teuk_default_params", "teuk_create", "teuk_destroy", "teuk_set_fields",
+    "teuk_get_fields", "teuk_set_source", "teuk_set_trajectory", "teuk_step", "teuk_sync",
+    "teuk_progress", "teuk_read_output", "teuk_read_psi", "teuk_enable_timing", "teuk_timing",
+    "teuk_footprint", "teuk_body", "teuk_strerror", "teuk_trajectory_schedule",
+]
+
+_lib = None
+
+
+def lib():
+    """Load libteuk.so (built by bh_ringdown_amd.build); raise if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libteuk not built: {LIB_PATH} missing "
+                           "(run python -c 'import __graft_entry__ as g; g.build()')")
+    l = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER(Params)
+    vp = ctypes.c_void_p
+    sig = {
+        "teuk_grid": ([P, _dp, _dp, _dp], ctypes.c_int),
+        "teuk_default_params": ([P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double],
+                                ctypes.c_int),
+        "teuk_create": ([P, _dp, _dp, ctypes.POINTER(vp)], ctypes.c_int),
+        "teuk_destroy": ([vp], None),
+        "teuk_set_fields": ([vp, _dp], ctypes.c_int),
+        "teuk_get_fields": ([vp, _dp], ctypes.c_int),
+        "teuk_set_source": ([vp, ctypes.POINTER(SourceStep), ctypes.c_int], ctypes.c_int),
+        "teuk_set_trajectory": ([vp, _dp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "teuk_step": ([vp, ctypes.c_int], ctypes.c_int),
+        "teuk_sync": ([vp], ctypes.c_int),
+        "teuk_progress": ([vp, ctypes.POINTER(ctypes.c_int), _dp], ctypes.c_int),
+        "teuk_read_output": ([vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OutputRow),
+                              ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "teuk_read_psi": ([vp, ctypes.c_int, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "teuk_enable_timing": ([vp, ctypes.c_int], ctypes.c_int),
+        "teuk_timing": ([vp, ctypes.POINTER(ctypes.c_longlong), _dp], ctypes.c_int),
+        "teuk_footprint": ([P, ctypes.c_int], ctypes.c_longlong),
+        "teuk_body": ([P, _dp, _dp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int],
+                      ctypes.c_int),
+        "teuk_strerror": ([ctypes.c_int], ctypes.c_char_p),
+        "teuk_trajectory_schedule": ([_dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                      ctypes.POINTER(SourceStep)], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(l, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = l
+    return l
+
+
+def strerror(code):
+    return lib().teuk_strerror(int(code)).decode()
+
+
+def _check(rc, what):
+    if rc != TEUK_OK:
+        raise TeukError(rc, what)
+
+
+def _ptr(a):
+    if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("arrays must be C-contiguous float64")
+    return a.ctypes.data_as(_dp)
+
+
+def default_params(N, M, xmin, xmax, a, m, dt_over_dx=0.5, T=0.0, **kw):
+    p = Params()
+    _check(lib().teuk_default_params(ctypes.byref(p), N, M, xmin, xmax, a, float(m), dt_over_dx, T),
+           "teuk_default_params")
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError(k)
+        setattr(p, k, v)
+    return p
+
+
+def grid(p):
+    """main.c:79-198 grid(): (x_c[N], r_c[N], theta[M])."""
+    x = np.zeros(p.N); r = np.zeros(p.N); th = np.zeros(p.M)
+    _check(lib().teuk_grid(ctypes.byref(p), _ptr(x), _ptr(r), _ptr(th)), "teuk_grid")
+    return x, r, th
+
+
+def trajectory_schedule(rows, nsteps, dt, start=0.0):
+    """The per-step source scalars body.cu:97-129 derives from .traj rows."""
+    rows = np.ascontiguousarray(rows, dtype=np.float64)
+    out = (SourceStep * nsteps)()
+    _check(lib().teuk_trajectory_schedule(_ptr(rows), rows.shape[0], nsteps, start, dt, out),
+           "teuk_trajectory_schedule")
+    return list(out)
+
+
+def footprint(p, nsteps=None):
+    return int(lib().teuk_footprint(ctypes.byref(p), p.nsteps() if nsteps is None else nsteps))
+
+
+class Solver:
+    """One m-mode solve resident on the GPU (a libteuk handle)."""
+
+    def __init__(self, p, r_c=None, theta=None):
+        self.p = p
+        if r_c is None or theta is None:
+            _, r_c, theta = grid(p)
+        self.r_c, self.theta = np.ascontiguousarray(r_c), np.ascontiguousarray(theta)
+        h = ctypes.c_void_p()
+        _check(lib().teuk_create(ctypes.byref(p), _ptr(self.r_c), _ptr(self.theta), ctypes.byref(h)),
+               "teuk_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().teuk_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def shape(self):
+        return (4, self.p.M, self.p.N)
+
+    def set_fields(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if u.shape != self.shape:
+            raise ValueError(f"fields must have shape {self.shape}")
+        _check(lib().teuk_set_fields(self._h, _ptr(u)), "teuk_set_fields")
+
+    def get_fields(self):
+        u = np.zeros(self.shape)
+        _check(lib().teuk_get_fields(self._h, _ptr(u)), "teuk_get_fields")
+        return u
+
+    def set_source(self, steps):
+        if steps is None or len(steps) == 0:
+            _check(lib().teuk_set_source(self._h, None, 0), "teuk_set_source")
+            return
+        arr = (SourceStep * len(steps))(*steps)
+        _check(lib().teuk_set_source(self._h, arr, len(steps)), "teuk_set_source")
+
+    def set_trajectory(self, rows, nsteps=None):
+        rows = np.ascontiguousarray(rows, dtype=np.float64)
+        if rows.ndim != 2 or rows.shape[1] != 10:
+            raise ValueError("trajectory rows must be (n, 10)")
+        n = self.p.nsteps() if nsteps is None else nsteps
+        _check(lib().teuk_set_trajectory(self._h, _ptr(rows), rows.shape[0], n), "teuk_set_trajectory")
+
+    def step(self, n):
+        _check(lib().teuk_step(self._h, int(n)), "teuk_step")
+
+    def sync(self):
+        _check(lib().teuk_sync(self._h), "teuk_sync")
+
+    def progress(self):
+        k = ctypes.c_int(); t = ctypes.c_double()
+        _check(lib().teuk_progress(self._h, ctypes.byref(k), ctypes.byref(t)), "teuk_progress")
+        return k.value, t.value
+
+    def output(self, first=0, max_rows=1 << 20):
+        n_have = ctypes.c_int()
+        cap = max(1, min(max_rows, self.p.nsteps() // self.p.Fr + 1))
+        rows = (OutputRow * cap)()
+        _check(lib().teuk_read_output(self._h, first, cap, rows, ctypes.byref(n_have)), "teuk_read_output")
+        n = n_have.value
+        k = np.array([rows[i].k for i in range(n)], dtype=np.int64)
+        w = np.array([rows[i].write for i in range(n)], dtype=np.int64)
+        t = np.array([rows[i].t for i in range(n)])
+        dedt = np.array([rows[i].dedt for i in range(n)])
+        h = np.array([list(rows[i].h) for i in range(n)]).reshape(n, 14)
+        return {"k": k, "write": w, "t": t, "dedt": dedt, "h": h}
+
+    def psi(self, first=0, max_rows=1 << 20):
+        cap = max(1, min(max_rows, self.p.nsteps() // self.p.Fr + 1))
+        buf = np.zeros((cap, 2, self.p.M))
+        n = ctypes.c_int()
+        _check(lib().teuk_read_psi(self._h, first, cap, _ptr(buf), ctypes.byref(n)), "teuk_read_psi")
+        return buf[: n.value].copy()
+
+    def enable_timing(self, on=True):
+        _check(lib().teuk_enable_timing(self._h, 1 if on else 0), "teuk_enable_timing")
+
+    def timing(self):
+        n = ctypes.c_longlong(); ms = ctypes.c_double()
+        _check(lib().teuk_timing(self._h, ctypes.byref(n), ctypes.byref(ms)), "teuk_timing")
+        return n.value, ms.value
+
+
+def body(p, r_c, theta, traj_path=None, out_path=None, full_path=None, verbose=False):
+    """body.cu:25-225 body(): one full m-mode solve with the reference's file I/O."""
+    enc = lambda s: s.encode() if s is not None else None
+    _check(lib().teuk_body(ctypes.byref(p), _ptr(np.ascontiguousarray(r_c)), _ptr(np.ascontiguousarray(theta)),
+                           enc(traj_path), enc(out_path), enc(full_path), 1 if verbose else 0), "teuk_body")

@@ -1,0 +1,54 @@
+"""Named workloads of the 2+1D Teukolsky solver and their synthetic inputs.
+
+Each workload fixes the grid (rho domain = the reference's [Xmin, Xmax]),
+spin, mode, step count and initial data.  Sizes are the ones the solver is
+measured and checked on.
+"""
+import math
+
+import numpy as np
+
+from . import teuk
+
+
+def j_at(p, rho):
+    """Extraction column closest to rho (the reference extracts at N-1)."""
+    return int(round((rho - p.xmin) / p.dx))
+
+
+def gaussian_pulse(p, seed, center=50.0, width=5.0):
+    """Complex Gaussian Psi = A e^{i phi0} exp(-(rho-c)^2/(2 w^2)) sin^4(theta),
+    Pi = 0, with (A, phi0) drawn from the seed (A ~ U[0.5, 1.5), phi0 ~ U[0, 2 pi))."""
+    rng = np.random.default_rng(seed)
+    amp = 0.5 + rng.random()
+    phase = 2.0 * math.pi * rng.random()
+    x = p.xmin + np.arange(p.N) * p.dx
+    th = np.arange(p.M) * (teuk.PI15 / p.M)
+    prof = np.exp(-((x - center) ** 2) / (2.0 * width * width))
+    ang = np.sin(th) ** 4
+    g = amp * np.outer(ang, prof)
+    u = np.zeros((4, p.M, p.N))
+    u[0] = g * math.cos(phase)
+    u[1] = g * math.sin(phase)
+    return u
+
+
+def plain_1024x32(steps=2000, integrator=0, Fr=1):
+    """Host-oracle parity case: s=-2, m=2, a=0.5; N=1024 on rho in [-100,300],
+    M=32; dt = 0.5 dx; no source; Gaussian initial Psi; Psi at rho=250."""
+    p = teuk.default_params(1024, 32, -100.0, 300.0, 0.5, 2, dt_over_dx=0.5)
+    p.T = steps * p.dt * (1.0 + 1e-12)
+    p.Fr = Fr
+    p.integrator = integrator
+    p.j_extract = j_at(p, 250.0)
+    return p
+
+
+def single_16384x64(T=1500.0, integrator=0, Fr=2):
+    """Single full solve: s=-2, m=2, a=0.9; N=16384 on rho in [-400,1200],
+    M=64; dt = 0.5 dx to t=1500 (30720 steps); Psi at rho=1000."""
+    p = teuk.default_params(16384, 64, -400.0, 1200.0, 0.9, 2, dt_over_dx=0.5, T=T)
+    p.Fr = Fr
+    p.integrator = integrator
+    p.j_extract = j_at(p, 1000.0)
+    return p

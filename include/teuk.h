@@ -1,0 +1,152 @@
+/*
+ * libteuk -- MI355X-native FP64 2+1D Teukolsky evolution (spin weight s=-2,
+ * one azimuthal mode m) behind a plain C ABI.  No HIP, CUDA or torch types
+ * appear in these signatures: host buffers are plain double arrays, sizes are
+ * ints, and every entry point returns 0 or a negative TEUK_E* code.
+ *
+ * Grid layout is the reference's (TD_GPU_code/HighOrder/parm.h:13):
+ *   idx(b, a) = N*b + a,  b = theta row (0..M-1), a = rho column (0..N-1).
+ * A field state is 4 planes of N*M doubles: q_re, q_im, p_re, p_im
+ * (Psi and Pi of kern.cu:10-13).
+ *
+ * Threading: one solver handle is driven by one host thread at a time.
+ * Different handles are independent (each owns its device memory and a HIP
+ * stream), which is how batched and served solves are built.
+ */
+#ifndef TEUK_H
+#define TEUK_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------- */
+#define TEUK_OK          0
+#define TEUK_EINVAL     -1   /* bad argument or unsupported size            */
+#define TEUK_EHIP       -2   /* a HIP runtime call failed                    */
+#define TEUK_ENOMEM     -3   /* device or host allocation failed             */
+#define TEUK_EDIVERGED  -4   /* non-finite field detected by the sentinel    */
+#define TEUK_EIO        -5   /* trajectory or waveform file error            */
+#define TEUK_ERANGE     -6   /* trajectory exhausted / step beyond schedule  */
+
+/* ---- run parameters ---------------------------------------------------- */
+/* Field names follow parm.h:1-11 and main.c:55-61 of the reference. */
+typedef struct teuk_params {
+  int    N, M;          /* radial (rho) and angular (theta) points           */
+  double xmin, xmax;    /* rho domain, parm.h:3-4 (Xmin, Xmax)               */
+  double mass, a;       /* main.c:55-56 (mass, aa)                           */
+  double s;             /* spin weight; only -2 is supported (kern.cu:142)   */
+  double m;             /* azimuthal mode number (body() argument mm)        */
+  double dx, dtheta, dt;/* main.c:59-61; dtheta is 3.14159265358979/M        */
+  double T;             /* stop time, parm.h:8                               */
+  int    Fr;            /* output cadence in steps, parm.h:7                 */
+  int    ptsx, ptsy;    /* source patch size, parm.h:10-11                   */
+  int    integrator;    /* 0 = SSP-RK3 (reference), 1 = classic RK4           */
+  int    j_extract;     /* extraction column; -1 = N-1 (scri, kern.cu:750)   */
+} teuk_params;
+
+/* Per-step particle state seen by the source deposit: the values body.cu
+ * hands to `sourced` at body.cu:122-124 after reading two trajectory lines
+ * (body.cu:112-120).  `timer` is the solver time at the start of the step. */
+typedef struct teuk_source_step {
+  double timer, start;
+  double rp, tp, phip, E, lz, Q;
+  double drdt, d2rdt2, dthdt, d2thdt2, dphidt, d2phidt2;
+} teuk_source_step;
+
+/* One output record, written every Fr steps (body.cu:204-212).  `t` is the
+ * solver time before the increment (timer), `h` the 7 (h+, hx) pairs for
+ * l = |m| .. |m|+6 (body.cu:189-202), `dedt` the energy flux
+ * (body.cu:174-183), `write` 1 when body.cu:207 would write the row. */
+typedef struct teuk_output_row {
+  int    k;
+  int    write;
+  double t;
+  double dedt;
+  double h[14];
+} teuk_output_row;
+
+typedef struct teuk_solver teuk_solver;
+
+/* ---- host set-up --------------------------------------------------------- */
+/* main.c:79-198 grid(): x_c[N], r_c[N] (Boyer-Lindquist r by Newton
+ * inversion of r*(r) = x/Omega(x)), theta[M].  Replaces grid(). */
+int teuk_grid(const teuk_params *p, double *x_c, double *r_c, double *theta);
+
+/* Convenience: fill a teuk_params with the reference's derived quantities
+ * (dx = (xmax-xmin)/N, dtheta = Pi/M, main.c:59-60), dt = dt_over_dx*dx,
+ * s = -2, Fr = 1, ptsx = ptsy = 128, j_extract = -1. */
+int teuk_default_params(teuk_params *p, int N, int M, double xmin, double xmax,
+                        double a, double m, double dt_over_dx, double T);
+
+/* ---- solver lifecycle ---------------------------------------------------- */
+/* Allocates device state, uploads r_c/theta, computes the Kerr coefficient
+ * fields and zeroes the fields (kernel_init, kern.cu:84-169; body.cu:32-54).
+ * N must be a multiple of 64 and >= 128; M a multiple of 8 and >= 16. */
+int teuk_create(const teuk_params *p, const double *r_c, const double *theta,
+                teuk_solver **out);
+void teuk_destroy(teuk_solver *s);
+
+/* Host <-> device field transfer, layout 4 x N x M (see top). */
+int teuk_set_fields(teuk_solver *s, const double *u);
+int teuk_get_fields(teuk_solver *s, double *u);
+
+/* Source schedule: one entry per step (NULL / 0 = no source).  Replaces the
+ * per-step reads of body.cu:112-129. */
+int teuk_set_source(teuk_solver *s, const teuk_source_step *steps, int nsteps);
+
+/* Builds the source schedule from raw .traj rows (10 columns, the format of
+ * GenOT/WL_BL_RK.cc and body.cu:97) with the reference's consumption pattern:
+ * prime with one line then read until t >= start (body.cu:97-103), then two
+ * lines per step, r += 1e-6, accelerations by backward difference over dt
+ * with zero initial velocities (body.cu:112-129). */
+int teuk_set_trajectory(teuk_solver *s, const double *rows, int nrows, int nsteps);
+
+/* Host-only: the per-step schedule teuk_set_trajectory builds, written to
+ * out[nsteps] (start = solver start time, 0 for the reference's p = 1). */
+int teuk_trajectory_schedule(const double *rows, int nrows, int nsteps, double start, double dt,
+                             teuk_source_step *out);
+
+/* Advance nsteps full time steps (SSP-RK3: three fused stage kernels per
+ * step; RK4: four).  Asynchronous on the solver's stream; use teuk_sync. */
+int teuk_step(teuk_solver *s, int nsteps);
+int teuk_sync(teuk_solver *s);
+
+/* Number of completed steps and current solver time. */
+int teuk_progress(teuk_solver *s, int *k, double *t);
+
+/* Output records produced so far (every Fr steps).  Copies up to max rows
+ * starting at record `first`; *n receives the count copied. */
+int teuk_read_output(teuk_solver *s, int first, int max, teuk_output_row *rows, int *n);
+
+/* Psi at the extraction column (M complex values, re plane then im plane)
+ * recorded with each output row: psi[(row - first)][2][M]. */
+int teuk_read_psi(teuk_solver *s, int first, int max, double *psi, int *n);
+
+/* Kernel timing: when enabled, teuk_step brackets every stage-kernel launch
+ * with HIP events on the solver stream.  teuk_timing returns the number of
+ * timed stage launches and their summed duration in milliseconds. */
+int teuk_enable_timing(teuk_solver *s, int on);
+int teuk_timing(teuk_solver *s, long long *launches, double *ms);
+
+/* Device bytes held by one solver of these parameters for `nsteps` of
+ * schedule (admission control for batching/serving). */
+long long teuk_footprint(const teuk_params *p, int nsteps);
+
+/* ---- drop-in for body() ---------------------------------------------------- */
+/* body.cu:25-225: one full m-mode solve.  Reads the trajectory file
+ * (traj_path, may be NULL for no source), evolves from zero fields to T,
+ * writes the waveform file in the reference format "%lf %g x14" to out_path
+ * (NULL: no file) and, if full_path is non-NULL, the same rows at %.17g.
+ * Prints the reference's stdout lines when verbose. */
+int teuk_body(const teuk_params *p, const double *r_c, const double *theta,
+              const char *traj_path, const char *out_path, const char *full_path,
+              int verbose);
+
+/* Human-readable text for an error code. */
+const char *teuk_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

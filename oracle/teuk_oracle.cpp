@@ -1,0 +1,455 @@
+/*
+ * CPU oracle for the 2+1D Teukolsky hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Restates, in plain FP64 C++, the update loop of the reference solver
+ * TD_GPU_code/HighOrder.  Every function cites the reference lines it
+ * follows.  The product library never links this file; tests, the smoke
+ * check and bench.py's cpu_baseline leg load it as the checker.
+ *
+ * Parity build:  g++ -O2 -ffp-contract=off  (no FMA contraction, so sums are
+ *                evaluated in the order written, as in the reference source)
+ * Timing build:  g++ -O3 -fopenmp           (cpu_baseline leg of bench.py)
+ *
+ * Pinning (how each function is checked, in tests/):
+ *   orc_grid      bitwise against the reference main.c grid() compiled by
+ *                 oracle/Makefile into oracle/_ref, plus the Newton residual
+ *                 r*(r) = x/Omega to 1e-12 at any size.
+ *   orc_fact/orc_swsh/orc_project
+ *                 bitwise against the reference proj.c compiled into
+ *                 oracle/_ref, plus closed forms of -2Y_lm and orthonormality.
+ *   orc_coeffs    against an independent symbolic derivation of the Teukolsky
+ *                 operator in the compactified coordinates (tests/golden/
+ *                 coeff_derivation.py), plus the Schwarzschild/Omega=1 limit.
+ *   orc_rhs       stencil exactness on polynomials, KO annihilation of degree
+ *                 <= 7, assembly against a direct evaluation on smooth fields.
+ *   orc_bc_*      exact extrapolation formulas and ordering invariants.
+ *   orc_step      SSP-RK3 / RK4 amplification polynomials on linear modes.
+ *   orc_source    parity unpinned against the reference (kern.cu needs the
+ *                 CUDA compiler, absent here); pinned by the distributional
+ *                 canonical form and physical checks (see teuk_source.inc).
+ */
+#include "teuk_oracle.h"
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+/* parm.h:12 -- the 15-digit pi used for dtheta (main.c:60), cot (kern.cu:204)
+ * and the spherical harmonics (proj.c:33,43). */
+const double kPi15 = 3.14159265358979;
+/* kern.cu:46 */
+const double kThird = 0.33333333333333333;
+/* kern.cu:196 -- first-derivative weight literal (not exactly 1/60) */
+const double kD1 = 0.016666666666667;
+
+inline size_t plane(const orc_params *p) { return (size_t)p->N * (size_t)p->M; }
+
+inline int extract_col(const orc_params *p) {
+  return (p->j_extract < 0) ? p->N - 1 : p->j_extract;
+}
+
+}  // namespace
+
+extern "C" {
+
+/* ---------------------------------------------------------------- grid --- */
+/* main.c:79-198: uniform rho and theta grids, the hyperboloidal layer Omega,
+ * r* = rho/Omega, and a Newton solve for the Boyer-Lindquist r(r*) marching
+ * from the outer edge inward, each point warm-started from its neighbour. */
+void orc_grid(int N, int M, double xmin, double xmax, double mass, double a,
+              double *x_c, double *r_c, double *theta) {
+  const double dx = (xmax - xmin) / N;
+  const double dth = kPi15 / M;
+  for (int l = 0; l < N; l++) x_c[l] = xmin + (double)l * dx;
+  for (int k = 0; k < M; k++) theta[k] = (double)k * dth;
+
+  const double rin = xmin + (xmax - xmin) * 0.64;
+  const double outer = xmax + 1e-6;
+  std::vector<double> rstar(N);
+  for (int l = 0; l < N; l++) {
+    double om = 1.0;
+    if (!(x_c[l] < rin)) om = 1.0 - std::pow((x_c[l] - rin) / (outer - rin), 4);
+    rstar[l] = x_c[l] / om;
+  }
+
+  const double root = std::sqrt(mass * mass - a * a);
+  const double rplus = mass + root, rminus = mass - root;
+  if (mass == 0.0) {
+    for (int l = 0; l < N; l++) r_c[l] = rstar[l];
+    return;
+  }
+  const double tol = 1e-12;
+  double r = rstar[N - 1];
+  for (int l = N - 1; l >= 0; l--) {
+    double rel;
+    double rnext;
+    do {
+      /* dr/dr* = Delta/(r^2+a^2) */
+      const double slope = (r * r + a * a - 2.0 * mass * r) / (r * r + a * a);
+      double tort;
+      if (mass == a) {
+        tort = r + 2.0 * a * std::log(r / a - 1.0) - 2.0 * a * a / (r - a);
+      } else if (a == 0.0) {
+        tort = r + 2.0 * mass * std::log(r / (2.0 * mass) - 1.0);
+      } else {
+        tort = r + (rplus * rplus + a * a) / (rplus - rminus) * std::log(r / rplus - 1.0)
+                 - (rminus * rminus + a * a) / (rplus - rminus) * std::log(r / rminus - 1.0);
+      }
+      rnext = r + (rstar[l] - tort) * slope;
+      /* Deviation from main.c: where the iterate reaches the horizon (or
+       * NaN) the reference goes on to log(0) and fills the rest of the grid
+       * with NaN; here r is clamped to r+ (the r* -> -inf limit) instead.
+       * Wherever the reference result is finite the two agree bitwise. */
+      if (!(rnext > rplus)) { rnext = rplus; r = rnext; break; }
+      rel = std::fabs(rnext - r) / r;
+      r = rnext;
+    } while (rel > tol);
+    r_c[l] = rnext;
+  }
+}
+
+/* ---------------------------------------------------------- coefficients --- */
+/* kern.cu:84-169.  The compactified Teukolsky operator is written as
+ *   d_tau^2 Psi = A^tr d_tau d_rho Psi + A^rr d_rho^2 Psi
+ *               + A^yy (d_th^2 + cot d_th) Psi + B^tau d_tau Psi
+ *               + B^rho d_rho Psi + C Psi
+ * and reduced to first order with Pi = d_tau Psi + bb d_rho Psi (kern.cu:147-158). */
+void orc_coeffs(const orc_params *p, const double *theta, const double *r_c,
+                double *coef) {
+  const int N = p->N, M = p->M;
+  const size_t NM = plane(p);
+  std::memset(coef, 0, 9 * NM * sizeof(double));
+  const double mass = p->mass, spin = p->a, em = p->m;
+  const double rin = p->xmin + (p->xmax - p->xmin) * 0.64;
+  const double outer = p->xmax + 1e-6;
+  for (int k = 3; k < M - 3; k++) {
+    const double y = theta[k];
+    const double sn = std::sin(y), cs = std::cos(y);
+    const double cot = 1.0 / std::tan(y), csc = 1.0 / std::sin(y);
+    for (int j = 0; j < N; j++) {
+      const double x = p->xmin + (double)j * p->dx;
+      double om, om1, om2;
+      if (x < rin) {
+        om = 1.0; om1 = 0.0; om2 = 0.0;
+      } else {
+        om = 1.0 - std::pow((x - rin) / (outer - rin), 4);
+        om1 = -4.0 * std::pow(x - rin, 3) / std::pow(outer - rin, 4);
+        om2 = -12.0 * std::pow(x - rin, 2) / std::pow(outer - rin, 4);
+      }
+      const double L = om - x * om1;      /* Omega^2 times d(r*)/d(rho) */
+      const double R = om * r_c[j];       /* compactified radius */
+      const double a2 = spin * spin;
+      const double om_2 = om * om;
+      const double S = R * R + om_2 * a2;                 /* (r^2+a^2) Omega^2 */
+      const double D = R * R - 2.0 * R * mass * om + om_2 * a2; /* Delta Omega^2 */
+      const double L2 = L * L, L3 = L2 * L;
+      const double bir = -(((2.0 * L - om_2) * (S * S)) / L2) + a2 * D * (sn * sn);
+
+      const double Atr = (2.0 * (L - om_2) * (S * S)) / (bir * L2);
+      const double Arr = -((om_2 * (S * S)) / (bir * L2));
+      const double Ayy = -(D / bir);
+      const double R2 = R * R, R3 = R2 * R, R5 = R3 * R2;
+      const double a4 = a2 * a2;
+      const double om3 = om_2 * om, om4 = om_2 * om_2, om5 = om4 * om;
+      const double BtRe = -((om * (R3 * (4.0 * L2 * (R - 3.0 * mass * om) + 2.0 * L * R2 * om1
+                                          + R2 * om * om2 * x)
+                                   + 2.0 * R * (L2 * om_2 * (5.0 * R - 4.0 * mass * om)
+                                                - 3.0 * L3 * (R - 2.0 * mass * om)
+                                                + 2.0 * L * R2 * om_2 * om1
+                                                + R2 * om3 * om2 * x) * a2
+                                   + (-6.0 * L3 * om_2 + 6.0 * L2 * om4 + 2.0 * L * R * om4 * om1
+                                      + R * om5 * om2 * x) * a4))
+                            / (bir * L3 * R));
+      const double BrRe = -((om * (R5 * (2.0 * om * om1 - 2.0 * om1 * om1 * x + om * om2 * x)
+                                   + 2.0 * R3 * om_2 * (2.0 * om * om1 - 2.0 * om1 * om1 * x
+                                                        + om * om2 * x) * a2
+                                   + om4 * (6.0 * om_2 + 2.0 * om1 * om1 * x * (-R + 3.0 * x)
+                                            + om * (2.0 * R * om1 - 12.0 * om1 * x + R * om2 * x)) * a4
+                                   + 2.0 * L2 * R * (2 * R3 - 6.0 * R2 * mass * om
+                                                     + 5.0 * R * om_2 * a2
+                                                     - 4.0 * mass * om3 * a2)))
+                            / (bir * L3 * R));
+      const double CRe = (D * (2.0 * R2 + 6.0 * R * mass * om - 6.0 * om_2 * a2
+                               + 4.0 * R2 * (cot * cot) - 4.0 * em * R2 * cot * csc
+                               + (em * em) * R2 * (csc * csc)))
+                         / (bir * R2);
+      const double BtIm = (2.0 * spin * (em * (-(om_2 * S) + L * (R2 + 2.0 * R * mass * om + om_2 * a2))
+                                         - 2.0 * L * D * cs))
+                          / (bir * L);
+      const double BrIm = (-2.0 * em * om_2 * spin * S) / (bir * L);
+      const double CIm = (2.0 * em * om * spin * (R2 + 2.0 * R * mass * om - 3.0 * om_2 * a2)) / (bir * R);
+
+      const size_t id = (size_t)k * N + j;
+      const double bb = -(Atr + std::sqrt(Atr * Atr + 4.0 * Arr)) / 2.0;
+      coef[0 * NM + id] = bb;
+      coef[1 * NM + id] = Ayy;
+      coef[2 * NM + id] = Atr + bb;
+      coef[3 * NM + id] = -BtRe;
+      coef[4 * NM + id] = -BtIm;
+      coef[5 * NM + id] = BrRe - bb * BtRe;
+      coef[6 * NM + id] = BrIm - bb * BtIm;
+      coef[7 * NM + id] = -CRe;
+      coef[8 * NM + id] = -CIm;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------- rhs --- */
+/* kern.cu:171-299 (kernel_rhs1; kernel_rhs2 at kern.cu:452-580 is the same
+ * arithmetic on the stage arrays). */
+void orc_rhs(const orc_params *p, const double *coef, const double *theta,
+             const double *u, const double *src, double *rhs) {
+  const int N = p->N, M = p->M;
+  const size_t NM = plane(p);
+  const double *qr = u, *qi = u + NM, *pr = u + 2 * NM, *pi = u + 3 * NM;
+  const double *bb = coef, *cc = coef + NM, *ee = coef + 2 * NM;
+  const double *are = coef + 3 * NM, *aim = coef + 4 * NM;
+  const double *dre_c = coef + 5 * NM, *dim = coef + 6 * NM;
+  const double *vre = coef + 7 * NM, *vim = coef + 8 * NM;
+  const double over_dx = 1.0 / p->dx;
+  const double over_dth = 1.0 / p->dtheta;
+  const double over_dt = 1.0 / p->dt;
+  const double wy = 0.05 * kThird * over_dth;
+  const double wyy = 0.05 * kThird * kThird * over_dth * over_dth;
+  const double wx = kD1 * over_dx;
+
+  auto dx1 = [&](const double *f, size_t c) {
+    return (45.0 * f[c + 1] - 45.0 * f[c - 1] - 9.0 * f[c + 2] + 9.0 * f[c - 2]
+            + f[c + 3] - f[c - 3]) * wx;
+  };
+  auto ko8 = [&](const double *f, size_t c) {
+    return f[c + 4] + f[c - 4] - 8.0 * (f[c + 3] + f[c - 3]) + 70.0 * f[c]
+           - 56.0 * (f[c + 1] + f[c - 1]) + 28.0 * (f[c + 2] + f[c - 2]);
+  };
+  const size_t sN = (size_t)N;
+  auto dy1 = [&](const double *f, size_t c) {
+    return ((f[c + 3 * sN] - f[c - 3 * sN]) - 9.0 * (f[c + 2 * sN] - f[c - 2 * sN])
+            + 45.0 * (f[c + sN] - f[c - sN])) * wy;
+  };
+  auto dy2 = [&](const double *f, size_t c) {
+    return (2.0 * (f[c + 3 * sN] + f[c - 3 * sN]) - 27.0 * (f[c + 2 * sN] + f[c - 2 * sN])
+            + 270.0 * (f[c + sN] + f[c - sN]) - 490.0 * f[c]) * wyy;
+  };
+
+  const int nth = p->nthreads > 1 ? p->nthreads : 1;
+#pragma omp parallel for schedule(static) num_threads(nth) if (nth > 1)
+  for (int b = 3; b < M - 3; b++) {
+    const double ctan = std::tan(0.5 * kPi15 - theta[b]);
+    for (int a = 3; a < N - 3; a++) {
+      const size_t c = (size_t)b * N + a;
+      const double dbdx = dx1(bb, c);
+      const double dre = dre_c[c] - dbdx * ee[c];
+      const double llr = dy2(qr, c) + ctan * dy1(qr, c);
+      const double lli = dy2(qi, c) + ctan * dy1(qi, c);
+      const double qrx = dx1(qr, c), qix = dx1(qi, c);
+      const double prx = dx1(pr, c), pix = dx1(pi, c);
+      double dqr = 0.0, dqi = 0.0, dpr = 0.0, dpi = 0.0;
+      if (a >= 4 && a < N - 4) {
+        dqr = ko8(qr, c); dqi = ko8(qi, c); dpr = ko8(pr, c); dpi = ko8(pi, c);
+      }
+      dqr = 0.1 * dqr * 0.0625 * over_dt;
+      dqi = 0.1 * dqi * 0.0625 * over_dt;
+      dpr = 0.1 * dpr * 0.0625 * over_dt;
+      dpi = 0.1 * dpi * 0.0625 * over_dt;
+      const double sr = src ? src[c] : 0.0;
+      const double si = src ? src[NM + c] : 0.0;
+      rhs[c] = -bb[c] * qrx + pr[c] - dqr;
+      rhs[NM + c] = -bb[c] * qix + pi[c] - dqi;
+      rhs[2 * NM + c] = sr + ee[c] * prx - dpr + cc[c] * llr + dre * qrx - dim[c] * qix
+                        - are[c] * pr[c] + aim[c] * pi[c] - vre[c] * qr[c] + vim[c] * qi[c];
+      rhs[3 * NM + c] = si + ee[c] * pix - dpi + cc[c] * lli + dre * qix + dim[c] * qrx
+                        - are[c] * pi[c] - aim[c] * pr[c] - vre[c] * qi[c] - vim[c] * qr[c];
+    }
+  }
+}
+
+/* ------------------------------------------------------------ boundaries --- */
+/* kern.cu:320-409: theta edge rows for columns 3 <= a < N-3.  Even m:
+ * quadratic extrapolation chained inward-out; odd m: the six rows vanish. */
+void orc_bc_theta(const orc_params *p, double *u) {
+  const int N = p->N, M = p->M;
+  const size_t NM = plane(p);
+  const bool even = (((int)p->m) % 2) == 0;
+  for (int f = 0; f < 4; f++) {
+    double *g = u + f * NM;
+    auto at = [&](int b, int a) -> double & { return g[(size_t)b * N + a]; };
+    for (int a = 3; a < N - 3; a++) {
+      if (even) {
+        at(2, a) = (4.0 * at(3, a) - at(4, a)) * kThird;
+        at(1, a) = (4.0 * at(2, a) - at(3, a)) * kThird;
+        at(M - 3, a) = (4.0 * at(M - 4, a) - at(M - 5, a)) * kThird;
+        at(M - 2, a) = (4.0 * at(M - 3, a) - at(M - 4, a)) * kThird;
+        at(0, a) = (4.0 * at(1, a) - at(2, a)) * kThird;
+        at(M - 1, a) = (4.0 * at(M - 2, a) - at(M - 3, a)) * kThird;
+      } else {
+        at(0, a) = 0.0; at(M - 1, a) = 0.0; at(1, a) = 0.0;
+        at(2, a) = 0.0; at(M - 2, a) = 0.0; at(M - 3, a) = 0.0;
+      }
+    }
+  }
+}
+
+/* kern.cu:412-449: rho edge columns on every row, after the theta rows. */
+void orc_bc_rho(const orc_params *p, double *u) {
+  const int N = p->N, M = p->M;
+  const size_t NM = plane(p);
+  for (int f = 0; f < 4; f++) {
+    double *g = u + f * NM;
+    for (int b = 0; b < M; b++) {
+      double *r = g + (size_t)b * N;
+      r[2] = (4.0 * r[3] - r[4]) * kThird;
+      r[N - 3] = (4.0 * r[N - 4] - r[N - 5]) * kThird;
+      r[1] = (4.0 * r[2] - r[3]) * kThird;
+      r[N - 2] = (4.0 * r[N - 3] - r[N - 4]) * kThird;
+      r[0] = (4.0 * r[1] - r[2]) * kThird;
+      r[N - 1] = (4.0 * r[N - 2] - r[N - 3]) * kThird;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ step --- */
+/* body.cu:131-165 step order with kern.cu:301-318 (u1), :582-600 (u2),
+ * :602-620 (u3, in place).  Updates touch 3<=b<M-3, 3<=a<N-3 only; the
+ * boundary fills then define every other point.
+ * RK4 (classic, same RHS, source frozen over the step like the reference)
+ * is the alternative integrator. */
+void orc_step(const orc_params *p, const double *coef, const double *theta,
+              double *u, const double *src, double *work) {
+  const int N = p->N, M = p->M;
+  const size_t NM = plane(p);
+  const double dt = p->dt;
+  double *uc = work;             /* 4 planes */
+  double *rhs = work + 4 * NM;   /* 4 planes */
+  double *acc = work + 8 * NM;   /* 4 planes (RK4) */
+  const int nth = p->nthreads > 1 ? p->nthreads : 1;
+  auto interior = [&](auto &&fn) {
+#pragma omp parallel for collapse(2) schedule(static) num_threads(nth) if (nth > 1)
+    for (int f = 0; f < 4; f++)
+      for (int b = 3; b < M - 3; b++)
+        for (int a = 3; a < N - 3; a++) fn(f * NM + (size_t)b * N + a);
+  };
+  if (p->integrator == 0) {
+    orc_rhs(p, coef, theta, u, src, rhs);
+    interior([&](size_t c) { uc[c] = u[c] + dt * rhs[c]; });
+    orc_bc_theta(p, uc); orc_bc_rho(p, uc);
+    orc_rhs(p, coef, theta, uc, src, rhs);
+    interior([&](size_t c) { uc[c] = 0.25 * (3.0 * u[c] + uc[c] + dt * rhs[c]); });
+    orc_bc_theta(p, uc); orc_bc_rho(p, uc);
+    orc_rhs(p, coef, theta, uc, src, rhs);
+    interior([&](size_t c) { u[c] = kThird * (u[c] + 2.0 * uc[c] + 2.0 * dt * rhs[c]); });
+    orc_bc_theta(p, u); orc_bc_rho(p, u);
+  } else {
+    const double h2 = 0.5 * dt, h6 = dt / 6.0, h3 = dt / 3.0;
+    orc_rhs(p, coef, theta, u, src, rhs);
+    interior([&](size_t c) { uc[c] = u[c] + h2 * rhs[c]; acc[c] = u[c] + h6 * rhs[c]; });
+    orc_bc_theta(p, uc); orc_bc_rho(p, uc);
+    orc_rhs(p, coef, theta, uc, src, rhs);
+    interior([&](size_t c) { uc[c] = u[c] + h2 * rhs[c]; acc[c] = acc[c] + h3 * rhs[c]; });
+    orc_bc_theta(p, uc); orc_bc_rho(p, uc);
+    orc_rhs(p, coef, theta, uc, src, rhs);
+    interior([&](size_t c) { uc[c] = u[c] + dt * rhs[c]; acc[c] = acc[c] + h3 * rhs[c]; });
+    orc_bc_theta(p, uc); orc_bc_rho(p, uc);
+    orc_rhs(p, coef, theta, uc, src, rhs);
+    interior([&](size_t c) { u[c] = acc[c] + h6 * rhs[c]; });
+    orc_bc_theta(p, u); orc_bc_rho(p, u);
+  }
+}
+
+/* ---------------------------------------------------------- projection --- */
+/* proj.c:8-14 */
+double orc_fact(double n) {
+  double f = 1.0;
+  for (double c = 1.0; c <= n; c++) f = f * c;
+  return f;
+}
+
+/* proj.c:16-35: spin-weight -2 spherical harmonic by the Wigner sum. */
+double orc_swsh(double ell, double ml, double the) {
+  double lo = 0.0;
+  if ((ml - 2.0) > 0) lo = ml - 2.0;
+  double hi = ell - 2.0;
+  if ((ell - 2.0) > (ell + ml)) hi = ml + ell;
+  double sum = 0.0;
+  for (double t = lo; t <= hi; t++) {
+    double w = std::pow(-1.0, t) * std::sqrt(orc_fact(ell + ml) * orc_fact(ell - ml)
+                                             * orc_fact(ell + 2.0) * orc_fact(ell - 2.0));
+    w = w / (orc_fact(ell + ml - t) * orc_fact(ell - 2.0 - t) * orc_fact(t) * orc_fact(t + 2.0 - ml));
+    sum = sum + w * std::pow(std::cos(the / 2.0), 2.0 * ell + ml - 2.0 - 2.0 * t)
+                  * std::pow(std::sin(the / 2.0), 2.0 * t + 2.0 - ml);
+  }
+  return std::sqrt((2.0 * ell + 1.0) / 4.0 / kPi15) * sum;
+}
+
+/* proj.c:37-75: composite Boole rule over M/4 panels of the theta grid. */
+double orc_project(double *qq, int M, double l, double m) {
+  const double dth = kPi15 / M;
+  static const double w[5] = {7.0, 32.0, 12.0, 32.0, 7.0};
+  double h = 0.0;
+  qq[M] = qq[M - 1];
+  for (int i = 0; i < M / 4; i++) {
+    double th = 4.0 * dth * (double)i;
+    for (int j = 0; j < 5; j++) {
+      if (j > 0) th += dth;
+      h += w[j] * 4.0 * kPi15 * qq[4 * i + j] * std::sin(th) * dth * orc_swsh(l, m, th);
+    }
+  }
+  return 2.0 * h / 45.0;
+}
+
+/* ------------------------------------------------------------------- run --- */
+/* body.cu:105-215 without file I/O: source, step, extraction of the column,
+ * rectangle-rule double time integral, energy flux and the 14 projections. */
+int orc_run(const orc_params *p, const double *r_c, const double *theta,
+            double *u, int nsteps, const orc_src_step *src_steps,
+            double *psi_hist, double *h_hist, double *dedt_hist) {
+  const int N = p->N, M = p->M;
+  const size_t NM = plane(p);
+  std::vector<double> coef(9 * NM), work(12 * NM);
+  std::vector<double> src;
+  if (src_steps) src.assign(2 * NM, 0.0);
+  orc_coeffs(p, theta, r_c, coef.data());
+  const int je = extract_col(p);
+  std::vector<double> ire(M, 0.0), iim(M, 0.0), dre(M + 1, 0.0), dim(M + 1, 0.0);
+  const double dt = p->dt;
+  const int am = std::abs((int)p->m);
+  for (int k = 0; k < nsteps; k++) {
+    if (src_steps) orc_source(p, theta, r_c, &src_steps[k], src.data());
+    orc_step(p, coef.data(), theta, u, src_steps ? src.data() : nullptr, work.data());
+    const double rr = r_c[je];
+    double dedt = 0.0;
+    for (int q = 0; q < M; q++) {
+      const double pr = u[(size_t)q * N + je], pi = u[NM + (size_t)q * N + je];
+      if (psi_hist) {
+        psi_hist[(size_t)k * 2 * M + q] = pr;
+        psi_hist[(size_t)k * 2 * M + M + q] = pi;
+      }
+      ire[q] = ire[q] + pr * dt;
+      iim[q] = iim[q] + pi * dt;
+      const double psi2 = ire[q] * ire[q] + iim[q] * iim[q];
+      const double rho2 = rr * rr + std::pow(p->a * std::cos(theta[q]), 2);
+      const double dvol = 0.5 * p->dtheta * std::sin(theta[q]) * rr * rr;
+      dedt = dedt + psi2 * dvol * std::pow(rr, 6) / std::pow(rho2, 4);
+      dre[q] = dre[q] + ire[q] * dt;
+      dim[q] = dim[q] + iim[q] * dt;
+    }
+    if (dedt_hist) dedt_hist[k] = dedt;
+    if (h_hist) {
+      for (int l = 0; l < 7; l++) {
+        h_hist[(size_t)k * 14 + 2 * l] = orc_project(dre.data(), M, (double)(l + am), p->m);
+        h_hist[(size_t)k * 14 + 2 * l + 1] = -orc_project(dim.data(), M, (double)(l + am), p->m);
+      }
+    }
+    if (!std::isfinite(u[(size_t)(M / 2) * N + N / 2])) return -4;
+  }
+  return 0;
+}
+
+}  // extern "C"
+
+#include "teuk_source.inc"

@@ -1,0 +1,110 @@
+"""The C-ABI library without a GPU: it loads, exports what include/teuk.h
+declares, and its host-only entry points (grid, parameters, trajectory
+schedule, argument validation) behave like the reference driver."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from bh_ringdown_amd import teuk
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", fn)).read()
+            names |= set(re.findall(r"^\s*(?:const\s+)?[a-z_ ]+\**\s*\*?\s*(teuk_[a-z_0-9]+)\s*\(", txt, re.M))
+    return names
+
+
+def test_exports_every_declared_symbol():
+    names = declared()
+    assert "teuk_create" in names and "teuk_body" in names
+    l = teuk.lib()
+    for n in sorted(names):
+        assert hasattr(l, n), n
+    assert names == set(teuk.EXPORTS)
+
+
+def test_default_params_follow_reference_derivations():
+    p = teuk.default_params(16000, 256, -50.0, 50.0, 0.5, 2, dt_over_dx=0.32, T=1500.0)
+    assert p.dx == (50.0 - -50.0) / 16000        # main.c:59
+    assert p.dtheta == 3.14159265358979 / 256     # main.c:60
+    assert p.s == -2.0 and p.ptsx == 128 and p.ptsy == 128 and p.j_extract == -1
+
+
+@pytest.mark.parametrize("N,M,xmin,xmax,a", [(1024, 32, -100.0, 300.0, 0.5), (16384, 64, -400.0, 1200.0, 0.9)])
+def test_host_grid_equals_oracle(N, M, xmin, xmax, a):
+    p = teuk.default_params(N, M, xmin, xmax, a, 2)
+    x, r, th = teuk.grid(p)
+    xo, ro, tho = O.grid(O.make_params(N, M, xmin, xmax, a, 2))
+    assert np.array_equal(x, xo) and np.array_equal(r, ro) and np.array_equal(th, tho)
+
+
+def test_create_rejects_bad_arguments_before_touching_the_gpu():
+    bad = [
+        dict(N=100), dict(M=12), dict(M=20), dict(s=-1.0), dict(Fr=0), dict(integrator=2),
+        dict(j_extract=5000), dict(a=1.0), dict(m=1.5), dict(dt=0.0),
+    ]
+    for kw in bad:
+        p = teuk.default_params(1024, 32, -100.0, 300.0, 0.5, 2, T=10.0)
+        for k, v in kw.items():
+            setattr(p, k, v)
+        with pytest.raises(teuk.TeukError) as e:
+            teuk.Solver(p, np.zeros(max(p.N, 1)), np.zeros(max(p.M, 1)))
+        assert e.value.code == teuk.TEUK_EINVAL, kw
+
+
+def test_footprint_scales_with_grid():
+    p1 = teuk.default_params(4096, 32, -400.0, 1200.0, 0.5, 2, T=100.0)
+    p2 = teuk.default_params(8192, 64, -400.0, 1200.0, 0.5, 2, T=100.0)
+    f1, f2 = teuk.footprint(p1), teuk.footprint(p2)
+    assert 3.5 < f2 / f1 < 4.5
+    # 3 states x 4 fields + 9 coefficient planes, 8 B each
+    assert f1 >= 8 * (12 + 9) * 4096 * 32
+
+
+def synthetic_traj(n, dt, t0=0.0):
+    """Rows t, r, th, rdot, thdot, phidot, phi, E, Lz, Q at spacing dt/2."""
+    t = t0 + np.arange(n) * (dt / 2)
+    rows = np.zeros((n, 10))
+    rows[:, 0] = t
+    rows[:, 1] = 6.0 - 0.01 * t
+    rows[:, 2] = 1.2 + 0.001 * t
+    rows[:, 3] = -0.01 + 1e-4 * t ** 2
+    rows[:, 4] = 0.001 + 2e-5 * t
+    rows[:, 5] = 0.05 + 1e-3 * t
+    rows[:, 6] = 0.05 * t
+    rows[:, 7], rows[:, 8], rows[:, 9] = 0.95, 3.2, 0.7
+    return rows
+
+
+def test_trajectory_consumption_pattern():
+    """body.cu:97-129: prime 1 row, read until t >= start, 2 rows per step."""
+    dt = 0.1
+    rows = synthetic_traj(50, dt)
+    sched = teuk.trajectory_schedule(rows, 10, dt)
+    # first row t=0 primes, the do-while reads row 1 (t = dt/2 >= 0), then
+    # step k uses row 2k+3
+    vprev = np.zeros(3)
+    timer = 0.0
+    for k, st in enumerate(sched):
+        row = rows[2 * k + 3]
+        assert st.rp == row[1] + 1e-6
+        assert (st.tp, st.drdt, st.dthdt, st.dphidt, st.phip, st.E, st.lz, st.Q) == \
+            (row[2], row[3], row[4], row[5], row[6], row[7], row[8], row[9])
+        assert st.d2rdt2 == (row[3] - vprev[0]) / dt
+        assert st.d2thdt2 == (row[4] - vprev[1]) / dt
+        assert st.d2phidt2 == (row[5] - vprev[2]) / dt
+        assert st.timer == timer and st.start == 0.0
+        vprev = row[3:6]
+        timer += dt
+    with pytest.raises(teuk.TeukError) as e:
+        teuk.trajectory_schedule(rows, 30, dt)      # needs 2 + 60 rows
+    assert e.value.code == teuk.TEUK_ERANGE

@@ -294,6 +294,8 @@ int teuk_create(const teuk_params *p, const double *r_c, const double *theta, te
   d.A = nstates == 4 ? states + 12 * NM : nullptr;
   if ((rc = dalloc(s, &d.coef, 9 * NM))) return fail(rc);
   if ((rc = dalloc(s, &d.ctan, p->M))) return fail(rc);
+  if ((rc = dalloc(s, &d.col4, 4 * (size_t)p->N))) return fail(rc);
+  if ((rc = dalloc(s, &d.row4, 4 * (size_t)p->M))) return fail(rc);
   if ((rc = dalloc(s, &d.theta, p->M))) return fail(rc);
   if ((rc = dalloc(s, &d.r_c, p->N))) return fail(rc);
   if ((rc = dalloc(s, &d.patch, 2 * (size_t)p->ptsx * p->ptsy))) return fail(rc);
@@ -338,6 +340,15 @@ int teuk_create(const teuk_params *p, const double *r_c, const double *theta, te
     extw[p->M + q] = std::pow(rho2, 4);
   }
   extw[2 * p->M] = std::pow(rr, 6);
+  std::vector<double> col4(4 * (size_t)p->N), row4(4 * (size_t)p->M), ctan(p->M);
+  host_coeff_tables(p->N, p->M, p->xmin, p->xmax, p->dx, theta, col4.data(), row4.data(), ctan.data());
+  if (hipMemcpyAsync(d.col4, col4.data(), col4.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) !=
+          hipSuccess ||
+      hipMemcpyAsync(d.row4, row4.data(), row4.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) !=
+          hipSuccess ||
+      hipMemcpyAsync(d.ctan, ctan.data(), ctan.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) !=
+          hipSuccess)
+    return fail(TEUK_EHIP);
   std::vector<double> projw(nproj);
   host_projection_weights(p->M, p->m, projw.data());
   if (hipMemcpyAsync(d.extw, extw.data(), extw.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) !=

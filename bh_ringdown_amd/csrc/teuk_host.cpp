@@ -111,4 +111,34 @@ void host_projection_weights(int M, double m, double *w) {
   }
 }
 
+// Per-column layer factors (kern.cu:110-127) and per-row trigonometric
+// factors (kern.cu:132-145, 204), evaluated with the host libm so that the
+// device assembly of the coefficient fields is reproducible bit for bit.
+void host_coeff_tables(int N, int M, double xmin, double xmax, double dx, const double *theta,
+                       double *col4, double *row4, double *ctan) {
+  const double rin = xmin + (xmax - xmin) * 0.64;
+  const double outer = xmax + 1e-6;
+  for (int j = 0; j < N; j++) {
+    const double x = xmin + (double)j * dx;
+    double om = 1.0, om1 = 0.0, om2 = 0.0;
+    if (!(x < rin)) {
+      om = 1.0 - std::pow((x - rin) / (outer - rin), 4);
+      om1 = -4.0 * std::pow(x - rin, 3) / std::pow(outer - rin, 4);
+      om2 = -12.0 * std::pow(x - rin, 2) / std::pow(outer - rin, 4);
+    }
+    col4[4 * j] = x;
+    col4[4 * j + 1] = om;
+    col4[4 * j + 2] = om1;
+    col4[4 * j + 3] = om2;
+  }
+  for (int k = 0; k < M; k++) {
+    const double y = theta[k];
+    row4[4 * k] = std::sin(y);
+    row4[4 * k + 1] = std::cos(y);
+    row4[4 * k + 2] = 1.0 / std::tan(y);
+    row4[4 * k + 3] = 1.0 / std::sin(y);
+    ctan[k] = std::tan(0.5 * kPi15 - y);
+  }
+}
+
 }  // namespace teuk

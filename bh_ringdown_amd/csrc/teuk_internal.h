@@ -58,7 +58,9 @@ struct SrcStep {
 struct DevState {
   double *U, *C, *D, *A;   // field states (4 planes each); A only for RK4
   double *coef;            // 9 planes: bb cc ee a_re a_im dre d_im v_re v_im
-  double *ctan;            // [M] tan(Pi/2 - theta)
+  double *ctan;            // [M] tan(Pi/2 - theta), host libm
+  double *col4;            // [N][4] x, Omega, Omega', Omega'' (host libm)
+  double *row4;            // [M][4] sin, cos, cot, csc of theta (host libm)
   double *theta, *r_c;     // [M], [N]
   double *patch;           // [2][ptsy][ptsx] source patch
   int *patch_org;          // [2] {row0, col0}; row0 = INT_MIN when empty
@@ -85,6 +87,9 @@ void host_grid(int N, int M, double xmin, double xmax, double mass, double a,
                double *x_c, double *r_c, double *theta);
 double host_swsh(double ell, double ml, double the);
 void host_projection_weights(int M, double m, double *w /*[7][5*(M/4)][3]*/);
+// Transcendental inputs of the coefficient fields (kern.cu:110-127, 204).
+void host_coeff_tables(int N, int M, double xmin, double xmax, double dx, const double *theta,
+                       double *col4, double *row4, double *ctan);
 bool source_available();
 
 }  // namespace teuk

@@ -32,71 +32,67 @@ __device__ __forceinline__ void wave_sync() {
 
 // ---------------------------------------------------------------------------
 // Coefficient fields (kernel_init, kern.cu:84-169).  One thread per point.
+// The transcendental inputs come from host tables -- per rho column
+// (x, Omega, Omega', Omega'') and per theta row (sin, cos, cot, csc) -- and
+// the per-point assembly uses only correctly rounded IEEE operations without
+// contraction, so every coefficient equals the host computation bit for bit.
 // Plane 5 holds d_re until k_fix_dre folds in -dbdx*ee (kern.cu:192-197),
 // which is time independent and therefore computed once here.
-__global__ __launch_bounds__(256) void k_init_coeffs(Grid g, const double *__restrict__ theta,
+__global__ __launch_bounds__(256) void k_init_coeffs(Grid g, const double *__restrict__ col4,
+                                                     const double *__restrict__ row4,
                                                      const double *__restrict__ r_c,
-                                                     double *__restrict__ coef,
-                                                     double *__restrict__ ctan) {
+                                                     double *__restrict__ coef) {
   const size_t NM = (size_t)g.N * g.M;
   const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= NM) return;
   const int k = (int)(id / g.N), j = (int)(id - (size_t)k * g.N);
-  if (j == 0) ctan[k] = tan(0.5 * kPi15 - theta[k]);
   double out[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (k >= 3 && k < g.M - 3) {
     const double mass = g.mass, spin = g.a, em = g.m;
-    const double rin = g.xmin + (g.xmax - g.xmin) * 0.64;
-    const double outer = g.xmax + 1e-6;
-    const double y = theta[k];
-    const double sn = sin(y), cs = cos(y);
-    const double cot = 1.0 / tan(y), csc = 1.0 / sin(y);
-    const double x = g.xmin + (double)j * g.dx;
-    double om = 1.0, om1 = 0.0, om2 = 0.0;
-    if (!(x < rin)) {
-      const double w = outer - rin, w4 = w * w * w * w, u = x - rin;
-      om = 1.0 - (u / w) * (u / w) * (u / w) * (u / w);
-      om1 = -4.0 * u * u * u / w4;
-      om2 = -12.0 * u * u / w4;
-    }
+    const double sn = row4[4 * k], cs = row4[4 * k + 1], cot = row4[4 * k + 2], csc = row4[4 * k + 3];
+    const double x = col4[4 * j], om = col4[4 * j + 1], om1 = col4[4 * j + 2], om2 = col4[4 * j + 3];
     const double L = om - x * om1;
     const double R = om * r_c[j];
-    const double a2 = spin * spin, a4 = a2 * a2;
-    const double o2 = om * om, o3 = o2 * om, o4 = o2 * o2, o5 = o4 * om;
-    const double S = R * R + o2 * a2;
-    const double D = R * R - 2.0 * R * mass * om + o2 * a2;
+    const double a2 = spin * spin;
+    const double om_2 = om * om;
+    const double S = R * R + om_2 * a2;
+    const double D = R * R - 2.0 * R * mass * om + om_2 * a2;
     const double L2 = L * L, L3 = L2 * L;
-    const double R2 = R * R, R3 = R2 * R, R5 = R3 * R2;
-    const double bir = -(((2.0 * L - o2) * (S * S)) / L2) + a2 * D * (sn * sn);
-    const double Atr = (2.0 * (L - o2) * (S * S)) / (bir * L2);
-    const double Arr = -((o2 * (S * S)) / (bir * L2));
+    const double bir = -(((2.0 * L - om_2) * (S * S)) / L2) + a2 * D * (sn * sn);
+    const double Atr = (2.0 * (L - om_2) * (S * S)) / (bir * L2);
+    const double Arr = -((om_2 * (S * S)) / (bir * L2));
     const double Ayy = -(D / bir);
-    const double BtRe =
-        -((om * (R3 * (4.0 * L2 * (R - 3.0 * mass * om) + 2.0 * L * R2 * om1 + R2 * om * om2 * x) +
-                 2.0 * R *
-                     (L2 * o2 * (5.0 * R - 4.0 * mass * om) - 3.0 * L3 * (R - 2.0 * mass * om) +
-                      2.0 * L * R2 * o2 * om1 + R2 * o3 * om2 * x) *
-                     a2 +
-                 (-6.0 * L3 * o2 + 6.0 * L2 * o4 + 2.0 * L * R * o4 * om1 + R * o5 * om2 * x) * a4)) /
-          (bir * L3 * R));
-    const double g1 = 2.0 * om * om1 - 2.0 * om1 * om1 * x + om * om2 * x;
-    const double BrRe =
-        -((om * (R5 * g1 + 2.0 * R3 * o2 * g1 * a2 +
-                 o4 * (6.0 * o2 + 2.0 * om1 * om1 * x * (-R + 3.0 * x) +
-                       om * (2.0 * R * om1 - 12.0 * om1 * x + R * om2 * x)) *
-                     a4 +
-                 2.0 * L2 * R *
-                     (2.0 * R3 - 6.0 * R2 * mass * om + 5.0 * R * o2 * a2 - 4.0 * mass * o3 * a2))) /
-          (bir * L3 * R));
-    const double CRe = (D * (2.0 * R2 + 6.0 * R * mass * om - 6.0 * o2 * a2 + 4.0 * R2 * (cot * cot) -
-                             4.0 * em * R2 * cot * csc + (em * em) * R2 * (csc * csc))) /
-                       (bir * R2);
-    const double BtIm =
-        (2.0 * spin * (em * (-(o2 * S) + L * (R2 + 2.0 * R * mass * om + o2 * a2)) - 2.0 * L * D * cs)) /
-        (bir * L);
-    const double BrIm = (-2.0 * em * o2 * spin * S) / (bir * L);
-    const double CIm = (2.0 * em * om * spin * (R2 + 2.0 * R * mass * om - 3.0 * o2 * a2)) / (bir * R);
-    const double bb = -(Atr + sqrt(Atr * Atr + 4.0 * Arr)) / 2.0;
+    const double R2 = R * R, R3 = R2 * R, R5 = R3 * R2;
+    const double a4 = a2 * a2;
+    const double om3 = om_2 * om, om4 = om_2 * om_2, om5 = om4 * om;
+    const double BtRe = -((om * (R3 * (4.0 * L2 * (R - 3.0 * mass * om) + 2.0 * L * R2 * om1
+                                        + R2 * om * om2 * x)
+                                 + 2.0 * R * (L2 * om_2 * (5.0 * R - 4.0 * mass * om)
+                                              - 3.0 * L3 * (R - 2.0 * mass * om)
+                                              + 2.0 * L * R2 * om_2 * om1
+                                              + R2 * om3 * om2 * x) * a2
+                                 + (-6.0 * L3 * om_2 + 6.0 * L2 * om4 + 2.0 * L * R * om4 * om1
+                                    + R * om5 * om2 * x) * a4))
+                          / (bir * L3 * R));
+    const double BrRe = -((om * (R5 * (2.0 * om * om1 - 2.0 * om1 * om1 * x + om * om2 * x)
+                                 + 2.0 * R3 * om_2 * (2.0 * om * om1 - 2.0 * om1 * om1 * x
+                                                      + om * om2 * x) * a2
+                                 + om4 * (6.0 * om_2 + 2.0 * om1 * om1 * x * (-R + 3.0 * x)
+                                          + om * (2.0 * R * om1 - 12.0 * om1 * x + R * om2 * x)) * a4
+                                 + 2.0 * L2 * R * (2 * R3 - 6.0 * R2 * mass * om
+                                                   + 5.0 * R * om_2 * a2
+                                                   - 4.0 * mass * om3 * a2)))
+                          / (bir * L3 * R));
+    const double CRe = (D * (2.0 * R2 + 6.0 * R * mass * om - 6.0 * om_2 * a2
+                             + 4.0 * R2 * (cot * cot) - 4.0 * em * R2 * cot * csc
+                             + (em * em) * R2 * (csc * csc)))
+                       / (bir * R2);
+    const double BtIm = (2.0 * spin * (em * (-(om_2 * S) + L * (R2 + 2.0 * R * mass * om + om_2 * a2))
+                                       - 2.0 * L * D * cs))
+                        / (bir * L);
+    const double BrIm = (-2.0 * em * om_2 * spin * S) / (bir * L);
+    const double CIm = (2.0 * em * om * spin * (R2 + 2.0 * R * mass * om - 3.0 * om_2 * a2)) / (bir * R);
+    const double bb = -(Atr + __dsqrt_rn(Atr * Atr + 4.0 * Arr)) / 2.0;
     out[0] = bb;
     out[1] = Ayy;
     out[2] = Atr + bb;
@@ -481,7 +477,7 @@ bool source_available() { return kSourceAvailable; }
 hipError_t launch_init_coeffs(const Grid &g, const DevState &d, hipStream_t s) {
   const size_t NM = (size_t)g.N * g.M;
   const unsigned nb = (unsigned)((NM + 255) / 256);
-  hipLaunchKernelGGL(k_init_coeffs, dim3(nb), dim3(256), 0, s, g, d.theta, d.r_c, d.coef, d.ctan);
+  hipLaunchKernelGGL(k_init_coeffs, dim3(nb), dim3(256), 0, s, g, d.col4, d.row4, d.r_c, d.coef);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_fix_dre, dim3(nb), dim3(256), 0, s, g, d.coef);

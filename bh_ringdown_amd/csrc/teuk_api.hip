@@ -83,9 +83,12 @@ void flush_timing(teuk_solver *s) {
   s->ev_used = 0;
 }
 
-int enqueue_step(teuk_solver *s) {
+// One time step whose index is (device step counter + k_rel): source patch,
+// then the fused stages.  Output records are completed by launch_finalize
+// once per batch of steps.
+int enqueue_step(teuk_solver *s, int k_rel) {
   const int nstage = s->P.integrator == 0 ? 3 : 4;
-  if (s->has_src) HIPCHK(launch_source(s->g, s->d, s->stream));
+  if (s->has_src) HIPCHK(launch_source(s->g, s->d, k_rel, s->stream));
   for (int st = 1; st <= nstage; st++) {
     if (s->timing) {
       if (s->ev_used + 2 > s->ev.size()) {
@@ -100,14 +103,13 @@ int enqueue_step(teuk_solver *s) {
         }
       }
       HIPCHK(hipEventRecord(s->ev[s->ev_used], s->stream));
-      HIPCHK(launch_stage(s->g, s->d, s->P.integrator, st, s->has_src, s->stream));
+      HIPCHK(launch_stage(s->g, s->d, s->P.integrator, st, s->has_src, k_rel, s->stream));
       HIPCHK(hipEventRecord(s->ev[s->ev_used + 1], s->stream));
       s->ev_used += 2;
     } else {
-      HIPCHK(launch_stage(s->g, s->d, s->P.integrator, st, s->has_src, s->stream));
+      HIPCHK(launch_stage(s->g, s->d, s->P.integrator, st, s->has_src, k_rel, s->stream));
     }
   }
-  HIPCHK(launch_extract(s->g, s->d, s->stream));
   return TEUK_OK;
 }
 
@@ -119,7 +121,8 @@ int build_graph(teuk_solver *s, int steps) {
   hipGraph_t graph = nullptr;
   HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
   int rc = TEUK_OK;
-  for (int i = 0; i < steps && rc == TEUK_OK; i++) rc = enqueue_step(s);
+  for (int i = 0; i < steps && rc == TEUK_OK; i++) rc = enqueue_step(s, i);
+  if (rc == TEUK_OK && launch_finalize(s->g, s->d, steps, s->stream) != hipSuccess) rc = TEUK_EHIP;
   hipError_t e = hipStreamEndCapture(s->stream, &graph);
   if (rc != TEUK_OK) {
     if (graph) (void)hipGraphDestroy(graph);
@@ -243,10 +246,10 @@ long long teuk_footprint(const teuk_params *p, int nsteps) {
   const long long NM = (long long)p->N * p->M;
   const long long states = (p->integrator == 0 ? 3 : 4) * 4 * NM;
   const long long nrec = (long long)(std::max(nsteps, 0) / std::max(p->Fr, 1)) + 1;
-  long long bytes = 8 * (states + 9 * NM + p->N + 3 * (long long)p->M);
+  long long bytes = 8 * (states + 2 * NM + (kNK + 1) * (long long)p->N + (kNR + 1) * (long long)p->M);
   bytes += 8 * (2LL * p->ptsx * p->ptsy) + 8;
   bytes += 8 * (4LL * p->M + 2 * p->M + 1 + 7LL * 5 * (p->M / 4) * 3);
-  bytes += nrec * (16 * 8 + 2 * 4 + 2LL * p->M * 8);
+  bytes += nrec * (16 * 8 + 2 * 4 + 6LL * p->M * 8);
   bytes += (long long)nsteps * (long long)sizeof(SrcStep);
   return bytes;
 }
@@ -292,28 +295,32 @@ int teuk_create(const teuk_params *p, const double *r_c, const double *theta, te
   d.C = states + 4 * NM;
   d.D = states + 8 * NM;
   d.A = nstates == 4 ? states + 12 * NM : nullptr;
-  if ((rc = dalloc(s, &d.coef, 9 * NM))) return fail(rc);
-  if ((rc = dalloc(s, &d.ctan, p->M))) return fail(rc);
-  if ((rc = dalloc(s, &d.col4, 4 * (size_t)p->N))) return fail(rc);
-  if ((rc = dalloc(s, &d.row4, 4 * (size_t)p->M))) return fail(rc);
+  if ((rc = dalloc(s, &d.ktab, kNK * (size_t)p->N))) return fail(rc);
+  if ((rc = dalloc(s, &d.rtab, kNR * (size_t)p->M))) return fail(rc);
+  if ((rc = dalloc(s, &d.dbdx, NM))) return fail(rc);
+  if ((rc = dalloc(s, &d.scratch, NM))) return fail(rc);
   if ((rc = dalloc(s, &d.theta, p->M))) return fail(rc);
   if ((rc = dalloc(s, &d.r_c, p->N))) return fail(rc);
   if ((rc = dalloc(s, &d.patch, 2 * (size_t)p->ptsx * p->ptsy))) return fail(rc);
   if ((rc = dalloc(s, &d.patch_org, 2))) return fail(rc);
-  if ((rc = dalloc(s, &d.ext, 4 * (size_t)p->M))) return fail(rc);
-  if ((rc = dalloc(s, &d.extw, 2 * (size_t)p->M + 1))) return fail(rc);
+  double *extw = nullptr, *projw = nullptr;
+  if ((rc = dalloc(s, &d.ext.integ, 4 * (size_t)p->M))) return fail(rc);
+  if ((rc = dalloc(s, &d.ext.snap, (size_t)g.nrec_cap * 6 * p->M))) return fail(rc);
+  if ((rc = dalloc(s, &d.ext.rec, (size_t)g.nrec_cap * 16))) return fail(rc);
+  if ((rc = dalloc(s, &d.ext.rec_k, (size_t)g.nrec_cap * 2))) return fail(rc);
+  if ((rc = dalloc(s, &extw, 2 * (size_t)p->M + 1))) return fail(rc);
   const size_t nproj = 7 * (size_t)(5 * (p->M / 4)) * 3;
-  if ((rc = dalloc(s, &d.projw, nproj))) return fail(rc);
-  if ((rc = dalloc(s, &d.rec, (size_t)g.nrec_cap * 16))) return fail(rc);
-  if ((rc = dalloc(s, &d.rec_k, (size_t)g.nrec_cap * 2))) return fail(rc);
-  if ((rc = dalloc(s, &d.psi, (size_t)g.nrec_cap * 2 * p->M))) return fail(rc);
+  if ((rc = dalloc(s, &projw, nproj))) return fail(rc);
+  d.ext.extw = extw;
+  d.ext.projw = projw;
   if ((rc = dalloc(s, &d.ctl, 1))) return fail(rc);
   d.steps = nullptr;
   d.nsteps_src = 0;
 
   // Zero fields and integrals (kern.cu:164-167, body.cu:86-91).
   if (hipMemsetAsync(states, 0, nstates * 4 * NM * sizeof(double), s->stream) != hipSuccess) return fail(TEUK_EHIP);
-  if (hipMemsetAsync(d.ext, 0, 4 * (size_t)p->M * sizeof(double), s->stream) != hipSuccess) return fail(TEUK_EHIP);
+  if (hipMemsetAsync(d.ext.integ, 0, 4 * (size_t)p->M * sizeof(double), s->stream) != hipSuccess)
+    return fail(TEUK_EHIP);
   if (hipMemsetAsync(d.patch, 0, 2 * (size_t)p->ptsx * p->ptsy * sizeof(double), s->stream) != hipSuccess)
     return fail(TEUK_EHIP);
   Ctl c0{};
@@ -332,30 +339,27 @@ int teuk_create(const teuk_params *p, const double *r_c, const double *theta, te
     return fail(TEUK_EHIP);
 
   // Extraction weights, body.cu:175-183 (rr is the extraction radius).
-  std::vector<double> extw(2 * p->M + 1);
+  std::vector<double> h_extw(2 * p->M + 1);
   const double rr = r_c[g.je];
   for (int q = 0; q < p->M; q++) {
     const double rho2 = rr * rr + std::pow(p->a * std::cos(theta[q]), 2);
-    extw[q] = 0.5 * p->dtheta * std::sin(theta[q]) * rr * rr;
-    extw[p->M + q] = std::pow(rho2, 4);
+    h_extw[q] = 0.5 * p->dtheta * std::sin(theta[q]) * rr * rr;
+    h_extw[p->M + q] = std::pow(rho2, 4);
   }
-  extw[2 * p->M] = std::pow(rr, 6);
-  std::vector<double> col4(4 * (size_t)p->N), row4(4 * (size_t)p->M), ctan(p->M);
-  host_coeff_tables(p->N, p->M, p->xmin, p->xmax, p->dx, theta, col4.data(), row4.data(), ctan.data());
-  if (hipMemcpyAsync(d.col4, col4.data(), col4.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) !=
+  h_extw[2 * p->M] = std::pow(rr, 6);
+  std::vector<double> ktab(kNK * (size_t)p->N), rtab(kNR * (size_t)p->M);
+  host_coeff_tables(p->N, p->M, p->xmin, p->xmax, p->dx, p->mass, p->a, p->m, r_c, theta, ktab.data(),
+                    rtab.data());
+  std::vector<double> h_projw(nproj);
+  host_projection_weights(p->M, p->m, h_projw.data());
+  if (hipMemcpyAsync(d.ktab, ktab.data(), ktab.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) !=
           hipSuccess ||
-      hipMemcpyAsync(d.row4, row4.data(), row4.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) !=
+      hipMemcpyAsync(d.rtab, rtab.data(), rtab.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) !=
           hipSuccess ||
-      hipMemcpyAsync(d.ctan, ctan.data(), ctan.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) !=
+      hipMemcpyAsync(extw, h_extw.data(), h_extw.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) !=
+          hipSuccess ||
+      hipMemcpyAsync(projw, h_projw.data(), h_projw.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) !=
           hipSuccess)
-    return fail(TEUK_EHIP);
-  std::vector<double> projw(nproj);
-  host_projection_weights(p->M, p->m, projw.data());
-  if (hipMemcpyAsync(d.extw, extw.data(), extw.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) !=
-      hipSuccess)
-    return fail(TEUK_EHIP);
-  if (hipMemcpyAsync(d.projw, projw.data(), projw.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) !=
-      hipSuccess)
     return fail(TEUK_EHIP);
   if (launch_init_coeffs(g, d, s->stream) != hipSuccess) return fail(TEUK_EHIP);
   if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(TEUK_EHIP);
@@ -435,8 +439,15 @@ int teuk_step(teuk_solver *s, int nsteps) {
   if (s->k_host + nsteps > s->nt) return TEUK_ERANGE;
   if (s->has_src && s->k_host + nsteps > s->d.nsteps_src) return TEUK_ERANGE;
   int rc = TEUK_OK;
+  // Direct launches of n steps, closed by one finalize.
+  auto direct = [&](int n) {
+    int r = TEUK_OK;
+    for (int i = 0; i < n && r == TEUK_OK; i++) r = enqueue_step(s, i);
+    if (r == TEUK_OK && n > 0 && launch_finalize(s->g, s->d, n, s->stream) != hipSuccess) r = TEUK_EHIP;
+    return r;
+  };
   if (s->timing) {
-    for (int i = 0; i < nsteps && rc == TEUK_OK; i++) rc = enqueue_step(s);
+    rc = direct(nsteps);
   } else {
     const int G = 32;
     int left = nsteps;
@@ -447,7 +458,7 @@ int teuk_step(teuk_solver *s, int nsteps) {
         left -= G;
       }
     }
-    for (int i = 0; i < left && rc == TEUK_OK; i++) rc = enqueue_step(s);
+    if (rc == TEUK_OK) rc = direct(left);
   }
   if (rc == TEUK_OK) s->k_host += nsteps;
   return rc;
@@ -481,8 +492,8 @@ int teuk_read_output(teuk_solver *s, int first, int max, teuk_output_row *rows, 
   if (cnt > 0) {
     std::vector<double> rec((size_t)cnt * 16);
     std::vector<int> rk((size_t)cnt * 2);
-    HIPCHK(hipMemcpy(rec.data(), s->d.rec + (size_t)first * 16, rec.size() * sizeof(double), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(rk.data(), s->d.rec_k + (size_t)first * 2, rk.size() * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(rec.data(), s->d.ext.rec + (size_t)first * 16, rec.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(rk.data(), s->d.ext.rec_k + (size_t)first * 2, rk.size() * sizeof(int), hipMemcpyDeviceToHost));
     for (int i = 0; i < cnt; i++) {
       rows[i].k = rk[2 * i];
       rows[i].write = rk[2 * i + 1];
@@ -501,9 +512,11 @@ int teuk_read_psi(teuk_solver *s, int first, int max, double *psi, int *n) {
   HIPCHK(hipStreamSynchronize(s->stream));
   HIPCHK(hipMemcpy(&c, s->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
   const int cnt = std::max(0, std::min(max, c.nrec - first));
+  // Each snapshot holds Psi (re, im) followed by the two time integrals.
+  const size_t M = (size_t)s->P.M;
   if (cnt > 0)
-    HIPCHK(hipMemcpy(psi, s->d.psi + (size_t)first * 2 * s->P.M, (size_t)cnt * 2 * s->P.M * sizeof(double),
-                     hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy2D(psi, 2 * M * sizeof(double), s->d.ext.snap + (size_t)first * 6 * M, 6 * M * sizeof(double),
+                       2 * M * sizeof(double), (size_t)cnt, hipMemcpyDeviceToHost));
   if (n) *n = cnt;
   return TEUK_OK;
 }

@@ -111,13 +111,18 @@ void host_projection_weights(int M, double m, double *w) {
   }
 }
 
-// Per-column layer factors (kern.cu:110-127) and per-row trigonometric
-// factors (kern.cu:132-145, 204), evaluated with the host libm so that the
-// device assembly of the coefficient fields is reproducible bit for bit.
-void host_coeff_tables(int N, int M, double xmin, double xmax, double dx, const double *theta,
-                       double *col4, double *row4, double *ctan) {
+// Coefficient fields of kernel_init (kern.cu:110-158) split into what depends
+// on the rho column only and what depends on the theta row only.  The layer
+// function Omega(rho) and its derivatives (kern.cu:119-127), the compactified
+// radius eR = Omega r and the Kerr factors are folded into 12 constants per
+// column; the device then needs one division and one square root per point.
+// Evaluated with the host libm so that the device assembly reproduces the
+// host computation bit for bit.
+void host_coeff_tables(int N, int M, double xmin, double xmax, double dx, double mass, double a,
+                       double m, const double *r_c, const double *theta, double *ktab, double *rtab) {
   const double rin = xmin + (xmax - xmin) * 0.64;
   const double outer = xmax + 1e-6;
+  const double spin = a, em = m;
   for (int j = 0; j < N; j++) {
     const double x = xmin + (double)j * dx;
     double om = 1.0, om1 = 0.0, om2 = 0.0;
@@ -126,18 +131,49 @@ void host_coeff_tables(int N, int M, double xmin, double xmax, double dx, const 
       om1 = -4.0 * std::pow(x - rin, 3) / std::pow(outer - rin, 4);
       om2 = -12.0 * std::pow(x - rin, 2) / std::pow(outer - rin, 4);
     }
-    col4[4 * j] = x;
-    col4[4 * j + 1] = om;
-    col4[4 * j + 2] = om1;
-    col4[4 * j + 3] = om2;
+    const double L = om - x * om1;  // Omega^2 d(r*)/d(rho)
+    const double R = om * r_c[j];   // eR of kern.cu:130
+    const double a2 = spin * spin, a4 = a2 * a2;
+    const double om_2 = om * om, om3 = om_2 * om, om4 = om_2 * om_2, om5 = om4 * om;
+    const double S = R * R + om_2 * a2;
+    const double D = R * R - 2.0 * R * mass * om + om_2 * a2;
+    const double L2 = L * L, L3 = L2 * L;
+    const double R2 = R * R, R3 = R2 * R, R5 = R3 * R2;
+    double K[kNK];
+    K[kK1] = ((2.0 * L - om_2) * (S * S)) / L2;
+    K[kK2] = a2 * D;
+    K[kKA] = (2.0 * (L - om_2) * (S * S)) / L2;
+    K[kKR] = -((om_2 * (S * S)) / L2);
+    K[kKD] = D;
+    K[kKBt] = -((om * (R3 * (4.0 * L2 * (R - 3.0 * mass * om) + 2.0 * L * R2 * om1 + R2 * om * om2 * x) +
+                       2.0 * R *
+                           (L2 * om_2 * (5.0 * R - 4.0 * mass * om) - 3.0 * L3 * (R - 2.0 * mass * om) +
+                            2.0 * L * R2 * om_2 * om1 + R2 * om3 * om2 * x) *
+                           a2 +
+                       (-6.0 * L3 * om_2 + 6.0 * L2 * om4 + 2.0 * L * R * om4 * om1 + R * om5 * om2 * x) * a4)) /
+                (L3 * R));
+    K[kKBr] = -((om * (R5 * (2.0 * om * om1 - 2.0 * om1 * om1 * x + om * om2 * x) +
+                       2.0 * R3 * om_2 * (2.0 * om * om1 - 2.0 * om1 * om1 * x + om * om2 * x) * a2 +
+                       om4 * (6.0 * om_2 + 2.0 * om1 * om1 * x * (-R + 3.0 * x) +
+                              om * (2.0 * R * om1 - 12.0 * om1 * x + R * om2 * x)) *
+                           a4 +
+                       2.0 * L2 * R * (2.0 * R3 - 6.0 * R2 * mass * om + 5.0 * R * om_2 * a2 - 4.0 * mass * om3 * a2))) /
+                (L3 * R));
+    K[kKC0] = (D * (2.0 * R2 + 6.0 * R * mass * om - 6.0 * om_2 * a2)) / R2;
+    K[kKI0] = (2.0 * spin * (em * (-(om_2 * S) + L * (R2 + 2.0 * R * mass * om + om_2 * a2)))) / L;
+    K[kKI1] = 4.0 * spin * D;
+    K[kKBi] = (-2.0 * em * om_2 * spin * S) / L;
+    K[kKCi] = (2.0 * em * om * spin * (R2 + 2.0 * R * mass * om - 3.0 * om_2 * a2)) / R;
+    for (int i = 0; i < kNK; i++) ktab[(size_t)i * N + j] = K[i];
   }
   for (int k = 0; k < M; k++) {
     const double y = theta[k];
-    row4[4 * k] = std::sin(y);
-    row4[4 * k + 1] = std::cos(y);
-    row4[4 * k + 2] = 1.0 / std::tan(y);
-    row4[4 * k + 3] = 1.0 / std::sin(y);
-    ctan[k] = std::tan(0.5 * kPi15 - y);
+    const double sn = std::sin(y), cs = std::cos(y);
+    const double cot = 1.0 / std::tan(y), csc = 1.0 / std::sin(y);
+    rtab[k] = sn * sn;
+    rtab[M + k] = cs;
+    rtab[2 * M + k] = 4.0 * (cot * cot) - 4.0 * em * cot * csc + (em * em) * (csc * csc);
+    rtab[3 * M + k] = std::tan(0.5 * kPi15 - y);  // kern.cu:204
   }
 }
 

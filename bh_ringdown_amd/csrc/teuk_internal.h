@@ -25,6 +25,15 @@ constexpr int kRows = 8;
 constexpr int kHalo = 4;            // KO reach in rho (kern.cu:247-256)
 constexpr int kLdsCols = kWave + 2 * kHalo;
 
+// Per-column constants of the coefficient fields (kern.cu:132-145 with the
+// theta dependence factored out).  With bir = K2 sin^2 - K1 and ib = 1/bir:
+//   A^tr = KA ib, A^rr = KR ib, A^yy = -KD ib, B^tau_re = KBt ib,
+//   B^rho_re = KBr ib, C_re = (KC0 + KD Y) ib, B^tau_im = (KI0 - KI1 cos) ib,
+//   B^rho_im = KBi ib, C_im = KCi ib,  Y = 4 cot^2 - 4 m cot csc + m^2 csc^2.
+enum { kK1, kK2, kKA, kKR, kKD, kKBt, kKBr, kKC0, kKI0, kKI1, kKBi, kKCi, kNK };
+// Per-row values: sin^2, cos, Y, tan(Pi/2 - theta); stored [4][M].
+constexpr int kNR = 4;
+
 // Scalar run description passed by value to every kernel.
 struct Grid {
   int N, M;
@@ -47,6 +56,16 @@ struct Ctl {
   double timer;        // solver time, advanced by repeated addition (body.cu:214)
 };
 
+// Extraction state and output records (device pointers).
+struct Ext {
+  double *integ;       // [4][M] int Psi dt (re, im), int int Psi dt (re, im)
+  double *snap;        // [nrec_cap][6][M] Psi, int, int int at output steps
+  double *rec;         // [nrec_cap][16] t, dE/dt, h[14]
+  int *rec_k;          // [nrec_cap][2] k, write flag
+  const double *extw;  // [2M+1] dvol, rho2^4, rr^6 (body.cu:181-183)
+  const double *projw; // [7][5*(M/4)][3] Boole/swsh factors per (l, term)
+};
+
 // Per-step particle scalars on the device (mirrors teuk_source_step).
 struct SrcStep {
   double timer, start;
@@ -57,39 +76,35 @@ struct SrcStep {
 // Device pointers of one solver.
 struct DevState {
   double *U, *C, *D, *A;   // field states (4 planes each); A only for RK4
-  double *coef;            // 9 planes: bb cc ee a_re a_im dre d_im v_re v_im
-  double *ctan;            // [M] tan(Pi/2 - theta), host libm
-  double *col4;            // [N][4] x, Omega, Omega', Omega'' (host libm)
-  double *row4;            // [M][4] sin, cos, cot, csc of theta (host libm)
+  double *ktab;            // [kNK][N] column constants
+  double *rtab;            // [kNR][M] row values
+  double *dbdx;            // [M][N] rho derivative of bb (kern.cu:192-196)
+  double *scratch;         // [M][N] set-up scratch (bb plane)
   double *theta, *r_c;     // [M], [N]
   double *patch;           // [2][ptsy][ptsx] source patch
-  int *patch_org;          // [2] {row0, col0}; row0 = INT_MIN when empty
+  int *patch_org;          // [2] {row0, col0}
   SrcStep *steps;          // [nsteps] source schedule, or null
   int nsteps_src;
-  double *ext;             // [4][M] int_re, int_im, dint_re, dint_im
-  double *extw;            // [M] dvol, [M] rho2^4, [1] rr^6
-  double *projw;           // [7][5*(M/4)][3] Boole/swsh factors per (l, term)
-  double *rec;             // [nrec_cap][2+14] t, dedt, h[14]
-  int *rec_k;              // [nrec_cap][2] k, write flag
-  double *psi;             // [nrec_cap][2][M]
+  Ext ext;
   Ctl *ctl;
 };
 
 // Launchers (teuk_kernels.hip).
 hipError_t launch_init_coeffs(const Grid &g, const DevState &d, hipStream_t s);
-hipError_t launch_source(const Grid &g, const DevState &d, hipStream_t s);
+hipError_t launch_source(const Grid &g, const DevState &d, int k_rel, hipStream_t s);
 hipError_t launch_stage(const Grid &g, const DevState &d, int integrator, int stage,
-                        bool has_src, hipStream_t s);
-hipError_t launch_extract(const Grid &g, const DevState &d, hipStream_t s);
+                        bool has_src, int k_rel, hipStream_t s);
+hipError_t launch_finalize(const Grid &g, const DevState &d, int n, hipStream_t s);
+bool source_available();
 
 // Host helpers (teuk_host.cpp).
 void host_grid(int N, int M, double xmin, double xmax, double mass, double a,
                double *x_c, double *r_c, double *theta);
 double host_swsh(double ell, double ml, double the);
 void host_projection_weights(int M, double m, double *w /*[7][5*(M/4)][3]*/);
-// Transcendental inputs of the coefficient fields (kern.cu:110-127, 204).
-void host_coeff_tables(int N, int M, double xmin, double xmax, double dx, const double *theta,
-                       double *col4, double *row4, double *ctan);
-bool source_available();
+// Column constants [kNK][N] and row values [kNR][M] of the coefficient fields.
+void host_coeff_tables(int N, int M, double xmin, double xmax, double dx, double mass, double a,
+                       double m, const double *r_c, const double *theta, double *ktab,
+                       double *rtab);
 
 }  // namespace teuk

@@ -1,11 +1,19 @@
 // HIP kernels for the FP64 2+1D Teukolsky evolution on gfx950 (MI355X).
 //
 // Hot kernel: k_stage<INTEG, STAGE> -- one Runge-Kutta stage fused end to end:
-// 6th-order rho/theta stencils, Kreiss-Oliger dissipation, the RHS assembly,
-// the stage combination and both boundary fills, in a single pass over the
-// grid.  It replaces the reference's kernel_rhs1/2 + kernel_update1/2/3 +
-// kernel_boundary1a/1b/2a/2b sequence (TD_GPU_code/HighOrder/kern.cu:171-753,
-// launched 12 times per step at body.cu:131-165).
+// Kerr coefficients evaluated in registers, 6th-order rho/theta stencils,
+// Kreiss-Oliger dissipation, the RHS assembly, the stage combination, both
+// boundary fills and (last stage) the extraction-column bookkeeping, in one
+// pass over the grid.  It replaces the reference's kernel_rhs1/2 +
+// kernel_update1/2/3 + kernel_boundary1a/1b/2a/2b sequence and the host loop
+// of body.cu:168-187 (TD_GPU_code/HighOrder/kern.cu:171-753, launched 12
+// times per step at body.cu:131-165).
+//
+// Data per point and stage: the stage input (4 doubles, read once), the step
+// base for stages 2-3 (4 doubles), the output (4 doubles) and the
+// time-independent dbdx plane (1 double).  The nine coefficient fields of
+// kernel_init are not stored: each is rebuilt per point from 13 per-column and
+// 3 per-row values with one division and one square root.
 //
 // Tile: a wavefront owns 64 consecutive rho columns (coalesced, rho is the
 // contiguous index of parm.h:13) and a chunk of 8 theta rows.  Psi's theta
@@ -17,6 +25,10 @@
 // 3,4 / M-5,M-4 (kern.cu:320-409) and the first and last rho wavefront fill
 // columns 0..2 / N-3..N-1 from lanes 3,4 / N-5,N-4 by cross-lane shuffles
 // (kern.cu:412-449), keeping the reference's theta-then-rho order.
+//
+// All arithmetic is built without FMA contraction and with the host's libm
+// values fed in through tables, so results equal the host evaluation of the
+// same expressions bit for bit.
 #include "teuk_internal.h"
 
 #include <climits>
@@ -30,97 +42,64 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The nine coefficient fields at one point (kern.cu:132-158) from the
+// column constants K[13] and the row values sin^2, cos and Y.  d_re is
+// returned without the -dbdx*ee term.
+struct Coef {
+  double bb, cc, ee, are, aim, dre, dim, vre, vim;
+};
+
+__device__ __forceinline__ Coef coef_point(const double (&K)[kNK], double sn2, double cs, double Y) {
+  const double bir = K[kK2] * sn2 - K[kK1];
+  const double ib = 1.0 / bir;
+  const double Atr = K[kKA] * ib, Arr = K[kKR] * ib;
+  const double BtRe = K[kKBt] * ib, BrRe = K[kKBr] * ib;
+  const double CRe = (K[kKC0] + K[kKD] * Y) * ib;
+  const double BtIm = (K[kKI0] - K[kKI1] * cs) * ib;
+  Coef c;
+  c.bb = -0.5 * (Atr + __dsqrt_rn(Atr * Atr + 4.0 * Arr));
+  c.cc = -(K[kKD] * ib);
+  c.ee = Atr + c.bb;
+  c.are = -BtRe;
+  c.aim = -BtIm;
+  c.dre = BrRe - c.bb * BtRe;
+  c.dim = K[kKBi] * ib - c.bb * BtIm;
+  c.vre = -CRe;
+  c.vim = -(K[kKCi] * ib);
+  return c;
+}
+
 // ---------------------------------------------------------------------------
-// Coefficient fields (kernel_init, kern.cu:84-169).  One thread per point.
-// The transcendental inputs come from host tables -- per rho column
-// (x, Omega, Omega', Omega'') and per theta row (sin, cos, cot, csc) -- and
-// the per-point assembly uses only correctly rounded IEEE operations without
-// contraction, so every coefficient equals the host computation bit for bit.
-// Plane 5 holds d_re until k_fix_dre folds in -dbdx*ee (kern.cu:192-197),
-// which is time independent and therefore computed once here.
-__global__ __launch_bounds__(256) void k_init_coeffs(Grid g, const double *__restrict__ col4,
-                                                     const double *__restrict__ row4,
-                                                     const double *__restrict__ r_c,
-                                                     double *__restrict__ coef) {
+// bb on interior rows (kern.cu:147) into a scratch plane, then its 6th-order
+// rho derivative (kern.cu:192-196), which is time independent.
+__global__ __launch_bounds__(256) void k_init_bb(Grid g, const double *__restrict__ ktab,
+                                                 const double *__restrict__ rtab, double *__restrict__ bbp) {
   const size_t NM = (size_t)g.N * g.M;
   const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= NM) return;
   const int k = (int)(id / g.N), j = (int)(id - (size_t)k * g.N);
-  double out[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double bb = 0.0;
   if (k >= 3 && k < g.M - 3) {
-    const double mass = g.mass, spin = g.a, em = g.m;
-    const double sn = row4[4 * k], cs = row4[4 * k + 1], cot = row4[4 * k + 2], csc = row4[4 * k + 3];
-    const double x = col4[4 * j], om = col4[4 * j + 1], om1 = col4[4 * j + 2], om2 = col4[4 * j + 3];
-    const double L = om - x * om1;
-    const double R = om * r_c[j];
-    const double a2 = spin * spin;
-    const double om_2 = om * om;
-    const double S = R * R + om_2 * a2;
-    const double D = R * R - 2.0 * R * mass * om + om_2 * a2;
-    const double L2 = L * L, L3 = L2 * L;
-    const double bir = -(((2.0 * L - om_2) * (S * S)) / L2) + a2 * D * (sn * sn);
-    const double Atr = (2.0 * (L - om_2) * (S * S)) / (bir * L2);
-    const double Arr = -((om_2 * (S * S)) / (bir * L2));
-    const double Ayy = -(D / bir);
-    const double R2 = R * R, R3 = R2 * R, R5 = R3 * R2;
-    const double a4 = a2 * a2;
-    const double om3 = om_2 * om, om4 = om_2 * om_2, om5 = om4 * om;
-    const double BtRe = -((om * (R3 * (4.0 * L2 * (R - 3.0 * mass * om) + 2.0 * L * R2 * om1
-                                        + R2 * om * om2 * x)
-                                 + 2.0 * R * (L2 * om_2 * (5.0 * R - 4.0 * mass * om)
-                                              - 3.0 * L3 * (R - 2.0 * mass * om)
-                                              + 2.0 * L * R2 * om_2 * om1
-                                              + R2 * om3 * om2 * x) * a2
-                                 + (-6.0 * L3 * om_2 + 6.0 * L2 * om4 + 2.0 * L * R * om4 * om1
-                                    + R * om5 * om2 * x) * a4))
-                          / (bir * L3 * R));
-    const double BrRe = -((om * (R5 * (2.0 * om * om1 - 2.0 * om1 * om1 * x + om * om2 * x)
-                                 + 2.0 * R3 * om_2 * (2.0 * om * om1 - 2.0 * om1 * om1 * x
-                                                      + om * om2 * x) * a2
-                                 + om4 * (6.0 * om_2 + 2.0 * om1 * om1 * x * (-R + 3.0 * x)
-                                          + om * (2.0 * R * om1 - 12.0 * om1 * x + R * om2 * x)) * a4
-                                 + 2.0 * L2 * R * (2 * R3 - 6.0 * R2 * mass * om
-                                                   + 5.0 * R * om_2 * a2
-                                                   - 4.0 * mass * om3 * a2)))
-                          / (bir * L3 * R));
-    const double CRe = (D * (2.0 * R2 + 6.0 * R * mass * om - 6.0 * om_2 * a2
-                             + 4.0 * R2 * (cot * cot) - 4.0 * em * R2 * cot * csc
-                             + (em * em) * R2 * (csc * csc)))
-                       / (bir * R2);
-    const double BtIm = (2.0 * spin * (em * (-(om_2 * S) + L * (R2 + 2.0 * R * mass * om + om_2 * a2))
-                                       - 2.0 * L * D * cs))
-                        / (bir * L);
-    const double BrIm = (-2.0 * em * om_2 * spin * S) / (bir * L);
-    const double CIm = (2.0 * em * om * spin * (R2 + 2.0 * R * mass * om - 3.0 * om_2 * a2)) / (bir * R);
-    const double bb = -(Atr + __dsqrt_rn(Atr * Atr + 4.0 * Arr)) / 2.0;
-    out[0] = bb;
-    out[1] = Ayy;
-    out[2] = Atr + bb;
-    out[3] = -BtRe;
-    out[4] = -BtIm;
-    out[5] = BrRe - bb * BtRe;
-    out[6] = BrIm - bb * BtIm;
-    out[7] = -CRe;
-    out[8] = -CIm;
-  }
+    double K[kNK];
 #pragma unroll
-  for (int c = 0; c < 9; c++) coef[c * NM + id] = out[c];
+    for (int i = 0; i < kNK; i++) K[i] = ktab[(size_t)i * g.N + j];
+    bb = coef_point(K, rtab[k], rtab[g.M + k], rtab[2 * g.M + k]).bb;
+  }
+  bbp[id] = bb;
 }
 
-// dre = d_re - dbdx*ee with dbdx the 6th-order rho derivative of bb
-// (kern.cu:192-197); zero dbdx outside 3 <= a < N-3.
-__global__ __launch_bounds__(256) void k_fix_dre(Grid g, double *__restrict__ coef) {
+__global__ __launch_bounds__(256) void k_init_dbdx(Grid g, const double *__restrict__ bbp,
+                                                   double *__restrict__ dbdx) {
   const size_t NM = (size_t)g.N * g.M;
   const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= NM) return;
-  const int k = (int)(id / g.N), a = (int)(id - (size_t)k * g.N);
-  const double *bb = coef;
-  double dbdx = 0.0;
+  const int a = (int)(id % g.N);
+  double d = 0.0;
   if (a >= 3 && a < g.N - 3)
-    dbdx = (45.0 * bb[id + 1] - 45.0 * bb[id - 1] - 9.0 * bb[id + 2] + 9.0 * bb[id - 2] + bb[id + 3] -
-            bb[id - 3]) *
-           (kD1 * (1.0 / g.dx));
-  coef[5 * NM + id] = coef[5 * NM + id] - dbdx * coef[2 * NM + id];
+    d = (45.0 * bbp[id + 1] - 45.0 * bbp[id - 1] - 9.0 * bbp[id + 2] + 9.0 * bbp[id - 2] + bbp[id + 3] -
+         bbp[id - 3]) *
+        (kD1 * (1.0 / g.dx));
+  dbdx[id] = d;
 }
 
 // ---------------------------------------------------------------------------
@@ -134,12 +113,17 @@ __global__ __launch_bounds__(256) void k_fix_dre(Grid g, double *__restrict__ co
 //     STAGE 2: out = base + dt/2 L(in); acc += dt/3 L(in)
 //     STAGE 3: out = base + dt L(in);   acc += dt/3 L(in)
 //     STAGE 4: out = acc + dt/6 L(in)
+// The last stage also integrates Psi at the extraction column twice in time
+// (body.cu:177-187) and snapshots it every Fr steps.
 template <int INTEG, int STAGE>
 __global__ __launch_bounds__(256) void k_stage(Grid g, const double *__restrict__ in, const double *base,
-                                               double *out, double *acc, const double *__restrict__ coef,
-                                               const double *__restrict__ ctan,
+                                               double *out, double *acc, const double *__restrict__ ktab,
+                                               const double *__restrict__ rtab,
+                                               const double *__restrict__ dbdx,
                                                const double *__restrict__ patch,
-                                               const int *__restrict__ porg, int has_src, int ngroups) {
+                                               const int *__restrict__ porg, int has_src, int ngroups,
+                                               Ext ext, Ctl *ctl, int k_rel) {
+  constexpr bool kLast = (INTEG == 0 && STAGE == 3) || (INTEG == 1 && STAGE == 4);
   __shared__ double lds[4][4][kLdsCols];
   const int N = g.N, M = g.M;
   const size_t NM = (size_t)N * M;
@@ -167,6 +151,11 @@ __global__ __launch_bounds__(256) void k_stage(Grid g, const double *__restrict_
   const double wy = 0.05 * kThird * over_dth;
   const double wyy = 0.05 * kThird * kThird * over_dth * over_dth;
 
+  // Column constants of the coefficient fields, loaded once per chunk.
+  double K[kNK];
+#pragma unroll
+  for (int i = 0; i < kNK; i++) K[i] = ktab[(size_t)i * N + a];
+
   // Psi (q) theta window: rows b0-3 .. b0+10 of this column.
   double wqr[kRows + 6], wqi[kRows + 6];
 #pragma unroll
@@ -185,6 +174,12 @@ __global__ __launch_bounds__(256) void k_stage(Grid g, const double *__restrict_
   if (has_src) {
     it0 = porg[0];
     ip0 = porg[1];
+  }
+  int kstep = 0;
+  bool snap = false;
+  if (kLast) {
+    kstep = ctl->k + k_rel;
+    snap = (kstep % g.Fr) == 0 && kstep / g.Fr < g.nrec_cap;
   }
 
   double lo[4] = {0, 0, 0, 0}, hi[4] = {0, 0, 0, 0};
@@ -216,10 +211,32 @@ __global__ __launch_bounds__(256) void k_stage(Grid g, const double *__restrict_
       }
     }
   };
+  // Store one finished row; in the last stage the extraction-column lane
+  // also advances that row's time integrals (body.cu:177-187).
   auto store_row = [&](int b, const double (&v)[4]) {
     const size_t c = (size_t)b * N + a;
 #pragma unroll
     for (int f = 0; f < 4; f++) out[f * NM + c] = v[f];
+    if (kLast && a == g.je) {
+      const double ir = ext.integ[b] + v[0] * dt;
+      const double ii = ext.integ[M + b] + v[1] * dt;
+      const double dr = ext.integ[2 * M + b] + ir * dt;
+      const double di = ext.integ[3 * M + b] + ii * dt;
+      ext.integ[b] = ir;
+      ext.integ[M + b] = ii;
+      ext.integ[2 * M + b] = dr;
+      ext.integ[3 * M + b] = di;
+      if (!isfinite(v[0]) || !isfinite(v[1])) ctl->status = 1;
+      if (snap) {
+        double *sn = ext.snap + (size_t)(kstep / g.Fr) * 6 * M;
+        sn[b] = v[0];
+        sn[M + b] = v[1];
+        sn[2 * M + b] = ir;
+        sn[3 * M + b] = ii;
+        sn[4 * M + b] = dr;
+        sn[5 * M + b] = di;
+      }
+    }
   };
 
 #pragma unroll
@@ -257,7 +274,7 @@ __global__ __launch_bounds__(256) void k_stage(Grid g, const double *__restrict_
         return r[c + 4] + r[c - 4] - 8.0 * (r[c + 3] + r[c - 3]) + 70.0 * r[c] - 56.0 * (r[c + 1] + r[c - 1]) +
                28.0 * (r[c + 2] + r[c - 2]);
       };
-      const double ct = ctan[b];
+      const double ct = rtab[3 * M + b];
       const double qry = ((wqr[i + 6] - wqr[i]) - 9.0 * (wqr[i + 5] - wqr[i + 1]) + 45.0 * (wqr[i + 4] - wqr[i + 2])) * wy;
       const double qryy = (2.0 * (wqr[i + 6] + wqr[i]) - 27.0 * (wqr[i + 5] + wqr[i + 1]) +
                            270.0 * (wqr[i + 4] + wqr[i + 2]) - 490.0 * wqr[i + 3]) * wyy;
@@ -280,10 +297,8 @@ __global__ __launch_bounds__(256) void k_stage(Grid g, const double *__restrict_
       dpi = 0.1 * dpi * 0.0625 * over_dt;
 
       const size_t id = c0 + a;
-      const double bb = coef[id], cc = coef[NM + id], ee = coef[2 * NM + id];
-      const double are = coef[3 * NM + id], aim = coef[4 * NM + id];
-      const double dre = coef[5 * NM + id], dim = coef[6 * NM + id];
-      const double vre = coef[7 * NM + id], vim = coef[8 * NM + id];
+      const Coef cf = coef_point(K, rtab[b], rtab[M + b], rtab[2 * M + b]);
+      const double dre = cf.dre - dbdx[id] * cf.ee;
 
       double sr = 0.0, si = 0.0;
       const int sb = b - it0, sa = a - ip0;
@@ -293,10 +308,12 @@ __global__ __launch_bounds__(256) void k_stage(Grid g, const double *__restrict_
       }
 
       double r[4];
-      r[0] = -bb * qrx + pr0 - dqr;
-      r[1] = -bb * qix + pi0 - dqi;
-      r[2] = sr + ee * prx - dpr + cc * llr + dre * qrx - dim * qix - are * pr0 + aim * pi0 - vre * qr0 + vim * qi0;
-      r[3] = si + ee * pix - dpi + cc * lli + dre * qix + dim * qrx - are * pi0 - aim * pr0 - vre * qi0 - vim * qr0;
+      r[0] = -cf.bb * qrx + pr0 - dqr;
+      r[1] = -cf.bb * qix + pi0 - dqi;
+      r[2] = sr + cf.ee * prx - dpr + cf.cc * llr + dre * qrx - cf.dim * qix - cf.are * pr0 + cf.aim * pi0 -
+             cf.vre * qr0 + cf.vim * qi0;
+      r[3] = si + cf.ee * pix - dpi + cf.cc * lli + dre * qix + cf.dim * qrx - cf.are * pi0 - cf.aim * pr0 -
+             cf.vre * qi0 - cf.vim * qr0;
 
       const double u[4] = {qr0, qi0, pr0, pi0};
       if (INTEG == 0) {
@@ -386,81 +403,67 @@ __global__ __launch_bounds__(256) void k_stage(Grid g, const double *__restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Extraction (kernel_boundary2b's column copy, kern.cu:750-751) fused with
-// the host-side bookkeeping of body.cu:174-202: the rectangle-rule double
-// time integral, the energy flux and, every Fr steps, the 14 projections
-// onto -2Y_lm (proj.c:37-75).  Arithmetic uses explicit round-to-nearest
-// operations in the reference's evaluation order, so the records equal the
-// host computation bit for bit given equal field values.  Also advances the
-// step counter and the solver time.
-__global__ __launch_bounds__(256) void k_extract(Grid g, const double *__restrict__ U, double *__restrict__ ext,
-                                                 const double *__restrict__ extw,
-                                                 const double *__restrict__ projw, double *__restrict__ rec,
-                                                 int *__restrict__ rec_k, double *__restrict__ psi, Ctl *ctl) {
-  __shared__ double term[256];
-  __shared__ double dint[2][256];
-  __shared__ int bad;
-  const int q = threadIdx.x, M = g.M, N = g.N;
-  const size_t NM = (size_t)N * M;
-  const int k = ctl->k;
-  const double timer = ctl->timer;
-  const int r = ctl->nrec;
-  const bool out_step = (k % g.Fr) == 0 && r < g.nrec_cap;
+// Output records for a batch of n completed steps (body.cu:174-212): for each
+// step k with k % Fr == 0, the energy flux from the snapshot of int Psi dt
+// and the 14 projections of int int Psi dt onto -2Y_lm (proj.c:37-75); the
+// solver time advanced by repeated addition (body.cu:214).  Explicit
+// round-to-nearest operations in the reference's evaluation order make the
+// records equal the host computation bit for bit.  Then advances the step
+// counter by n.
+__global__ __launch_bounds__(256) void k_finalize(Grid g, Ext ext, Ctl *ctl, int n) {
+  const int M = g.M, t = threadIdx.x;
+  const int k0 = ctl->k;
+  const double timer0 = ctl->timer;
+  const int r_lo = (k0 + g.Fr - 1) / g.Fr;                 // first record in [k0, k0+n)
+  int r_hi = (k0 + n - 1) / g.Fr + 1;                       // one past the last
+  if (r_hi > g.nrec_cap) r_hi = g.nrec_cap;
+  const int nrec = r_hi > r_lo ? r_hi - r_lo : 0;
   const double dt = g.dt;
-  if (q == 0) bad = 0;
-  __syncthreads();
-  if (q < M) {
-    const double pr = U[(size_t)q * N + g.je], pi = U[NM + (size_t)q * N + g.je];
-    const double ir = __dadd_rn(ext[q], __dmul_rn(pr, dt));
-    const double ii = __dadd_rn(ext[M + q], __dmul_rn(pi, dt));
-    const double psi2 = __dadd_rn(__dmul_rn(ir, ir), __dmul_rn(ii, ii));
-    term[q] = __ddiv_rn(__dmul_rn(__dmul_rn(psi2, extw[q]), extw[2 * M]), extw[M + q]);
-    const double dr = __dadd_rn(ext[2 * M + q], __dmul_rn(ir, dt));
-    const double di = __dadd_rn(ext[3 * M + q], __dmul_rn(ii, dt));
-    ext[q] = ir;
-    ext[M + q] = ii;
-    ext[2 * M + q] = dr;
-    ext[3 * M + q] = di;
-    dint[0][q] = dr;
-    dint[1][q] = di;
-    if (!isfinite(pr) || !isfinite(pi)) bad = 1;
-    if (out_step) {
-      psi[((size_t)r * 2) * M + q] = pr;
-      psi[((size_t)r * 2 + 1) * M + q] = pi;
-    }
-  }
-  __syncthreads();
-  if (out_step) {
-    if (q == 0) {
+  const int nterm = 5 * (M / 4);
+  for (int task = t; task < nrec * 15; task += blockDim.x) {
+    const int r = r_lo + task / 15, part = task % 15;
+    const double *sn = ext.snap + (size_t)r * 6 * M;
+    double *rec = ext.rec + (size_t)r * 16;
+    if (part == 14) {
       double dedt = 0.0;
-      for (int i = 0; i < M; i++) dedt = __dadd_rn(dedt, term[i]);
-      rec[(size_t)r * 16] = timer;
-      rec[(size_t)r * 16 + 1] = dedt;
-      rec_k[2 * r] = k;
-      rec_k[2 * r + 1] = ((double)k * dt) >= (g.xmax + 200.0) ? 1 : 0;
-    }
-    if (q >= 64 && q < 64 + 14) {
-      const int l = (q - 64) >> 1, part = (q - 64) & 1;
-      const int nterm = 5 * (M / 4);
-      const double *wl = projw + (size_t)l * nterm * 3;
-      const double *dq = dint[part];
+      for (int q = 0; q < M; q++) {
+        const double ir = sn[2 * M + q], ii = sn[3 * M + q];
+        const double psi2 = __dadd_rn(__dmul_rn(ir, ir), __dmul_rn(ii, ii));
+        dedt = __dadd_rn(dedt, __ddiv_rn(__dmul_rn(__dmul_rn(psi2, ext.extw[q]), ext.extw[2 * M]),
+                                         ext.extw[M + q]));
+      }
+      rec[1] = dedt;
+    } else {
+      const int l = part >> 1, im = part & 1;
+      const double *wl = ext.projw + (size_t)l * nterm * 3;
+      const double *dq = sn + (4 + im) * M;
       double h = 0.0;
-      for (int t = 0; t < nterm; t++) {
-        const int pt = 4 * (t / 5) + (t % 5);
+      for (int tm = 0; tm < nterm; tm++) {
+        const int pt = 4 * (tm / 5) + (tm % 5);
         const double v = dq[pt < M ? pt : M - 1];
-        const double term = __dmul_rn(__dmul_rn(__dmul_rn(__dmul_rn(wl[3 * t], v), wl[3 * t + 1]), g.dtheta), wl[3 * t + 2]);
+        const double term =
+            __dmul_rn(__dmul_rn(__dmul_rn(__dmul_rn(wl[3 * tm], v), wl[3 * tm + 1]), g.dtheta), wl[3 * tm + 2]);
         h = __dadd_rn(h, term);
       }
       h = __ddiv_rn(__dmul_rn(2.0, h), 45.0);
-      rec[(size_t)r * 16 + 2 + (q - 64)] = part ? -h : h;
+      rec[2 + part] = im ? -h : h;
     }
   }
   __syncthreads();
-  if (q == 0) {
-    ctl->k = k + 1;
-    ctl->timer = timer + dt;
-    if (out_step) ctl->nrec = r + 1;
-    if (bad) ctl->status = 1;
+  if (t == 0) {
+    double timer = timer0;
+    for (int k = k0; k < k0 + n; k++) {
+      if (k % g.Fr == 0 && k / g.Fr < g.nrec_cap) {
+        const int r = k / g.Fr;
+        ext.rec[(size_t)r * 16] = timer;
+        ext.rec_k[2 * r] = k;
+        ext.rec_k[2 * r + 1] = ((double)k * dt) >= (g.xmax + 200.0) ? 1 : 0;
+      }
+      timer = __dadd_rn(timer, dt);
+    }
+    ctl->k = k0 + n;
+    ctl->timer = timer;
+    ctl->nrec = r_hi > ctl->nrec ? r_hi : ctl->nrec;
   }
 }
 
@@ -477,10 +480,10 @@ bool source_available() { return kSourceAvailable; }
 hipError_t launch_init_coeffs(const Grid &g, const DevState &d, hipStream_t s) {
   const size_t NM = (size_t)g.N * g.M;
   const unsigned nb = (unsigned)((NM + 255) / 256);
-  hipLaunchKernelGGL(k_init_coeffs, dim3(nb), dim3(256), 0, s, g, d.col4, d.row4, d.r_c, d.coef);
+  hipLaunchKernelGGL(k_init_bb, dim3(nb), dim3(256), 0, s, g, d.ktab, d.rtab, d.scratch);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_fix_dre, dim3(nb), dim3(256), 0, s, g, d.coef);
+  hipLaunchKernelGGL(k_init_dbdx, dim3(nb), dim3(256), 0, s, g, d.scratch, d.dbdx);
   return hipGetLastError();
 }
 
@@ -489,65 +492,45 @@ static inline int waves_per_block(int M) {
   return chunks >= 4 ? 4 : (chunks >= 2 ? 2 : 1);
 }
 
-hipError_t launch_stage(const Grid &g, const DevState &d, int integrator, int stage, bool has_src,
+hipError_t launch_stage(const Grid &g, const DevState &d, int integrator, int stage, bool has_src, int k_rel,
                         hipStream_t s) {
   const int wpb = waves_per_block(g.M);
   const int ngroups = g.M / (kRows * wpb);
   const int nb = (g.N / kWave) * ngroups;
   const dim3 grid(nb), block(kWave, wpb);
   const int hs = has_src ? 1 : 0;
+#define TEUK_STAGE(I, S, IN, BASE, OUT, ACC)                                                                    \
+  hipLaunchKernelGGL((k_stage<I, S>), grid, block, 0, s, g, IN, BASE, OUT, ACC, d.ktab, d.rtab, d.dbdx, d.patch, \
+                     d.patch_org, hs, ngroups, d.ext, d.ctl, k_rel)
   if (integrator == 0) {
     switch (stage) {
-      case 1:
-        hipLaunchKernelGGL((k_stage<0, 1>), grid, block, 0, s, g, d.U, d.U, d.C, nullptr, d.coef, d.ctan,
-                           d.patch, d.patch_org, hs, ngroups);
-        break;
-      case 2:
-        hipLaunchKernelGGL((k_stage<0, 2>), grid, block, 0, s, g, d.C, d.U, d.D, nullptr, d.coef, d.ctan,
-                           d.patch, d.patch_org, hs, ngroups);
-        break;
-      case 3:
-        hipLaunchKernelGGL((k_stage<0, 3>), grid, block, 0, s, g, d.D, d.U, d.U, nullptr, d.coef, d.ctan,
-                           d.patch, d.patch_org, hs, ngroups);
-        break;
-      default:
-        return hipErrorInvalidValue;
+      case 1: TEUK_STAGE(0, 1, d.U, d.U, d.C, nullptr); break;
+      case 2: TEUK_STAGE(0, 2, d.C, d.U, d.D, nullptr); break;
+      case 3: TEUK_STAGE(0, 3, d.D, d.U, d.U, nullptr); break;
+      default: return hipErrorInvalidValue;
     }
   } else {
     switch (stage) {
-      case 1:
-        hipLaunchKernelGGL((k_stage<1, 1>), grid, block, 0, s, g, d.U, d.U, d.C, d.A, d.coef, d.ctan,
-                           d.patch, d.patch_org, hs, ngroups);
-        break;
-      case 2:
-        hipLaunchKernelGGL((k_stage<1, 2>), grid, block, 0, s, g, d.C, d.U, d.D, d.A, d.coef, d.ctan,
-                           d.patch, d.patch_org, hs, ngroups);
-        break;
-      case 3:
-        hipLaunchKernelGGL((k_stage<1, 3>), grid, block, 0, s, g, d.D, d.U, d.C, d.A, d.coef, d.ctan,
-                           d.patch, d.patch_org, hs, ngroups);
-        break;
-      case 4:
-        hipLaunchKernelGGL((k_stage<1, 4>), grid, block, 0, s, g, d.C, d.U, d.U, d.A, d.coef, d.ctan,
-                           d.patch, d.patch_org, hs, ngroups);
-        break;
-      default:
-        return hipErrorInvalidValue;
+      case 1: TEUK_STAGE(1, 1, d.U, d.U, d.C, d.A); break;
+      case 2: TEUK_STAGE(1, 2, d.C, d.U, d.D, d.A); break;
+      case 3: TEUK_STAGE(1, 3, d.D, d.U, d.C, d.A); break;
+      case 4: TEUK_STAGE(1, 4, d.C, d.U, d.U, d.A); break;
+      default: return hipErrorInvalidValue;
     }
   }
+#undef TEUK_STAGE
   return hipGetLastError();
 }
 
-hipError_t launch_extract(const Grid &g, const DevState &d, hipStream_t s) {
-  hipLaunchKernelGGL(k_extract, dim3(1), dim3(256), 0, s, g, d.U, d.ext, d.extw, d.projw, d.rec, d.rec_k, d.psi,
-                     d.ctl);
+hipError_t launch_finalize(const Grid &g, const DevState &d, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, g, d.ext, d.ctl, n);
   return hipGetLastError();
 }
 
-hipError_t launch_source(const Grid &g, const DevState &d, hipStream_t s) {
+hipError_t launch_source(const Grid &g, const DevState &d, int k_rel, hipStream_t s) {
   const int cells = g.ptsx * g.ptsy;
   hipLaunchKernelGGL(k_source, dim3((cells + 255) / 256), dim3(256), 0, s, g, d.theta, d.r_c, d.steps,
-                     d.nsteps_src, d.patch, d.patch_org, d.ctl);
+                     d.nsteps_src, d.patch, d.patch_org, d.ctl, k_rel);
   return hipGetLastError();
 }
 

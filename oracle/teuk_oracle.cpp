@@ -128,74 +128,83 @@ void orc_coeffs(const orc_params *p, const double *theta, const double *r_c,
   const double mass = p->mass, spin = p->a, em = p->m;
   const double rin = p->xmin + (p->xmax - p->xmin) * 0.64;
   const double outer = p->xmax + 1e-6;
+  /* The theta dependence of kernel_init's formulas enters only through
+   * sin^2(theta) in bir, cos(theta) in B^tau_im and the angular potential
+   * Y = 4 cot^2 - 4 m cot csc + m^2 csc^2 in C_re.  Each field is rewritten
+   * as (column constant [+ column constant x row value]) / bir, with
+   * bir = K2 sin^2 - K1; the algebra is that of kern.cu:132-145 term by
+   * term, only regrouped. */
+  enum { K1, K2, KA, KR, KD, KBt, KBr, KC0, KI0, KI1, KBi, KCi, NK };
+  std::vector<double> K((size_t)NK * N);
+  for (int j = 0; j < N; j++) {
+    const double x = p->xmin + (double)j * p->dx;
+    double om, om1, om2;
+    if (x < rin) {
+      om = 1.0; om1 = 0.0; om2 = 0.0;
+    } else {
+      om = 1.0 - std::pow((x - rin) / (outer - rin), 4);
+      om1 = -4.0 * std::pow(x - rin, 3) / std::pow(outer - rin, 4);
+      om2 = -12.0 * std::pow(x - rin, 2) / std::pow(outer - rin, 4);
+    }
+    const double L = om - x * om1;      /* Omega^2 times d(r*)/d(rho) */
+    const double R = om * r_c[j];       /* compactified radius */
+    const double a2 = spin * spin, a4 = a2 * a2;
+    const double om_2 = om * om, om3 = om_2 * om, om4 = om_2 * om_2, om5 = om4 * om;
+    const double S = R * R + om_2 * a2;                 /* (r^2+a^2) Omega^2 */
+    const double D = R * R - 2.0 * R * mass * om + om_2 * a2; /* Delta Omega^2 */
+    const double L2 = L * L, L3 = L2 * L;
+    const double R2 = R * R, R3 = R2 * R, R5 = R3 * R2;
+    double *c = &K[0];
+    c[K1 * N + j] = ((2.0 * L - om_2) * (S * S)) / L2;
+    c[K2 * N + j] = a2 * D;
+    c[KA * N + j] = (2.0 * (L - om_2) * (S * S)) / L2;
+    c[KR * N + j] = -((om_2 * (S * S)) / L2);
+    c[KD * N + j] = D;
+    c[KBt * N + j] = -((om * (R3 * (4.0 * L2 * (R - 3.0 * mass * om) + 2.0 * L * R2 * om1 + R2 * om * om2 * x)
+                              + 2.0 * R * (L2 * om_2 * (5.0 * R - 4.0 * mass * om)
+                                           - 3.0 * L3 * (R - 2.0 * mass * om)
+                                           + 2.0 * L * R2 * om_2 * om1 + R2 * om3 * om2 * x) * a2
+                              + (-6.0 * L3 * om_2 + 6.0 * L2 * om4 + 2.0 * L * R * om4 * om1
+                                 + R * om5 * om2 * x) * a4))
+                       / (L3 * R));
+    c[KBr * N + j] = -((om * (R5 * (2.0 * om * om1 - 2.0 * om1 * om1 * x + om * om2 * x)
+                              + 2.0 * R3 * om_2 * (2.0 * om * om1 - 2.0 * om1 * om1 * x + om * om2 * x) * a2
+                              + om4 * (6.0 * om_2 + 2.0 * om1 * om1 * x * (-R + 3.0 * x)
+                                       + om * (2.0 * R * om1 - 12.0 * om1 * x + R * om2 * x)) * a4
+                              + 2.0 * L2 * R * (2.0 * R3 - 6.0 * R2 * mass * om + 5.0 * R * om_2 * a2
+                                                - 4.0 * mass * om3 * a2)))
+                       / (L3 * R));
+    c[KC0 * N + j] = (D * (2.0 * R2 + 6.0 * R * mass * om - 6.0 * om_2 * a2)) / R2;
+    c[KI0 * N + j] = (2.0 * spin * (em * (-(om_2 * S) + L * (R2 + 2.0 * R * mass * om + om_2 * a2)))) / L;
+    c[KI1 * N + j] = 4.0 * spin * D;
+    c[KBi * N + j] = (-2.0 * em * om_2 * spin * S) / L;
+    c[KCi * N + j] = (2.0 * em * om * spin * (R2 + 2.0 * R * mass * om - 3.0 * om_2 * a2)) / R;
+  }
   for (int k = 3; k < M - 3; k++) {
     const double y = theta[k];
     const double sn = std::sin(y), cs = std::cos(y);
     const double cot = 1.0 / std::tan(y), csc = 1.0 / std::sin(y);
+    const double sn2 = sn * sn;
+    const double Y = 4.0 * (cot * cot) - 4.0 * em * cot * csc + (em * em) * (csc * csc);
     for (int j = 0; j < N; j++) {
-      const double x = p->xmin + (double)j * p->dx;
-      double om, om1, om2;
-      if (x < rin) {
-        om = 1.0; om1 = 0.0; om2 = 0.0;
-      } else {
-        om = 1.0 - std::pow((x - rin) / (outer - rin), 4);
-        om1 = -4.0 * std::pow(x - rin, 3) / std::pow(outer - rin, 4);
-        om2 = -12.0 * std::pow(x - rin, 2) / std::pow(outer - rin, 4);
-      }
-      const double L = om - x * om1;      /* Omega^2 times d(r*)/d(rho) */
-      const double R = om * r_c[j];       /* compactified radius */
-      const double a2 = spin * spin;
-      const double om_2 = om * om;
-      const double S = R * R + om_2 * a2;                 /* (r^2+a^2) Omega^2 */
-      const double D = R * R - 2.0 * R * mass * om + om_2 * a2; /* Delta Omega^2 */
-      const double L2 = L * L, L3 = L2 * L;
-      const double bir = -(((2.0 * L - om_2) * (S * S)) / L2) + a2 * D * (sn * sn);
-
-      const double Atr = (2.0 * (L - om_2) * (S * S)) / (bir * L2);
-      const double Arr = -((om_2 * (S * S)) / (bir * L2));
-      const double Ayy = -(D / bir);
-      const double R2 = R * R, R3 = R2 * R, R5 = R3 * R2;
-      const double a4 = a2 * a2;
-      const double om3 = om_2 * om, om4 = om_2 * om_2, om5 = om4 * om;
-      const double BtRe = -((om * (R3 * (4.0 * L2 * (R - 3.0 * mass * om) + 2.0 * L * R2 * om1
-                                          + R2 * om * om2 * x)
-                                   + 2.0 * R * (L2 * om_2 * (5.0 * R - 4.0 * mass * om)
-                                                - 3.0 * L3 * (R - 2.0 * mass * om)
-                                                + 2.0 * L * R2 * om_2 * om1
-                                                + R2 * om3 * om2 * x) * a2
-                                   + (-6.0 * L3 * om_2 + 6.0 * L2 * om4 + 2.0 * L * R * om4 * om1
-                                      + R * om5 * om2 * x) * a4))
-                            / (bir * L3 * R));
-      const double BrRe = -((om * (R5 * (2.0 * om * om1 - 2.0 * om1 * om1 * x + om * om2 * x)
-                                   + 2.0 * R3 * om_2 * (2.0 * om * om1 - 2.0 * om1 * om1 * x
-                                                        + om * om2 * x) * a2
-                                   + om4 * (6.0 * om_2 + 2.0 * om1 * om1 * x * (-R + 3.0 * x)
-                                            + om * (2.0 * R * om1 - 12.0 * om1 * x + R * om2 * x)) * a4
-                                   + 2.0 * L2 * R * (2 * R3 - 6.0 * R2 * mass * om
-                                                     + 5.0 * R * om_2 * a2
-                                                     - 4.0 * mass * om3 * a2)))
-                            / (bir * L3 * R));
-      const double CRe = (D * (2.0 * R2 + 6.0 * R * mass * om - 6.0 * om_2 * a2
-                               + 4.0 * R2 * (cot * cot) - 4.0 * em * R2 * cot * csc
-                               + (em * em) * R2 * (csc * csc)))
-                         / (bir * R2);
-      const double BtIm = (2.0 * spin * (em * (-(om_2 * S) + L * (R2 + 2.0 * R * mass * om + om_2 * a2))
-                                         - 2.0 * L * D * cs))
-                          / (bir * L);
-      const double BrIm = (-2.0 * em * om_2 * spin * S) / (bir * L);
-      const double CIm = (2.0 * em * om * spin * (R2 + 2.0 * R * mass * om - 3.0 * om_2 * a2)) / (bir * R);
-
+      auto k_ = [&](int which) { return K[(size_t)which * N + j]; };
+      const double bir = k_(K2) * sn2 - k_(K1);
+      const double ib = 1.0 / bir;
+      const double Atr = k_(KA) * ib, Arr = k_(KR) * ib;
+      const double BtRe = k_(KBt) * ib, BrRe = k_(KBr) * ib;
+      const double CRe = (k_(KC0) + k_(KD) * Y) * ib;
+      const double BtIm = (k_(KI0) - k_(KI1) * cs) * ib;
+      const double bb = -0.5 * (Atr + std::sqrt(Atr * Atr + 4.0 * Arr));
       const size_t id = (size_t)k * N + j;
-      const double bb = -(Atr + std::sqrt(Atr * Atr + 4.0 * Arr)) / 2.0;
       coef[0 * NM + id] = bb;
-      coef[1 * NM + id] = Ayy;
+      coef[1 * NM + id] = -(k_(KD) * ib);          /* A^yy = -Delta/bir */
       coef[2 * NM + id] = Atr + bb;
       coef[3 * NM + id] = -BtRe;
       coef[4 * NM + id] = -BtIm;
       coef[5 * NM + id] = BrRe - bb * BtRe;
-      coef[6 * NM + id] = BrIm - bb * BtIm;
+      coef[6 * NM + id] = k_(KBi) * ib - bb * BtIm;
       coef[7 * NM + id] = -CRe;
-      coef[8 * NM + id] = -CIm;
+      coef[8 * NM + id] = -(k_(KCi) * ib);
     }
   }
 }

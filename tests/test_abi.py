@@ -66,8 +66,8 @@ def test_footprint_scales_with_grid():
     p2 = teuk.default_params(8192, 64, -400.0, 1200.0, 0.5, 2, T=100.0)
     f1, f2 = teuk.footprint(p1), teuk.footprint(p2)
     assert 3.5 < f2 / f1 < 4.5
-    # 3 states x 4 fields + 9 coefficient planes, 8 B each
-    assert f1 >= 8 * (12 + 9) * 4096 * 32
+    # 3 states x 4 fields + the dbdx plane + set-up scratch, 8 B each
+    assert f1 >= 8 * (12 + 2) * 4096 * 32
 
 
 def synthetic_traj(n, dt, t0=0.0):

@@ -314,8 +314,15 @@ int teuk_create(const teuk_params *p, const double *r_c, const double *theta, te
   d.ext.extw = extw;
   d.ext.projw = projw;
   if ((rc = dalloc(s, &d.ctl, 1))) return fail(rc);
-  d.steps = nullptr;
-  d.nsteps_src = 0;
+  if ((rc = dalloc(s, &d.phase, 2 * (size_t)p->N))) return fail(rc);
+  d.sched = nullptr;
+  d.nsched = 0;
+  {
+    std::vector<double> ph(2 * (size_t)p->N);
+    host_source_phase(p->N, p->mass, p->a, p->m, r_c, ph.data());
+    if (hipMemcpy(d.phase, ph.data(), ph.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(TEUK_EHIP);
+  }
 
   // Zero fields and integrals (kern.cu:164-167, body.cu:86-91).
   if (hipMemsetAsync(states, 0, nstates * 4 * NM * sizeof(double), s->stream) != hipSuccess) return fail(TEUK_EHIP);
@@ -401,13 +408,31 @@ int teuk_set_source(teuk_solver *s, const teuk_source_step *steps, int nsteps) {
   }
   if (!source_available()) return TEUK_EINVAL;
   static_assert(sizeof(teuk_source_step) == sizeof(SrcStep), "source step layout");
+  const SrcStep *raw = reinterpret_cast<const SrcStep *>(steps);
+  for (int k = 0; k < nsteps; k++)
+    if (!(raw[k].rp > s->g.mass + std::sqrt(s->g.mass * s->g.mass - s->g.a * s->g.a))) return TEUK_EINVAL;
+  // Host: per-step transcendentals and patch placement; device: the T_4
+  // coefficients of every step, one thread each.
+  std::vector<SrcPrep> prep((size_t)nsteps);
+  for (int k = 0; k < nsteps; k++) host_source_prepare(s->g, raw[k], prep[k]);
+  std::vector<teuk_t4::StepIn> hin((size_t)nsteps);
+  std::vector<SrcDev> hdev((size_t)nsteps);
+  for (int k = 0; k < nsteps; k++) {
+    hin[k] = prep[k].in;
+    hdev[k] = prep[k].dev;
+  }
   HIPCHK(hipStreamSynchronize(s->stream));
-  SrcStep *dst = nullptr;
+  SrcDev *dst = nullptr;
+  teuk_t4::StepIn *din = nullptr;
   int rc = dalloc(s, &dst, (size_t)nsteps);
   if (rc) return rc;
-  HIPCHK(hipMemcpy(dst, steps, (size_t)nsteps * sizeof(SrcStep), hipMemcpyHostToDevice));
-  s->d.steps = dst;
-  s->d.nsteps_src = nsteps;
+  if ((rc = dalloc(s, &din, (size_t)nsteps))) return rc;
+  HIPCHK(hipMemcpy(dst, hdev.data(), (size_t)nsteps * sizeof(SrcDev), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(din, hin.data(), (size_t)nsteps * sizeof(teuk_t4::StepIn), hipMemcpyHostToDevice));
+  HIPCHK(launch_source_coeffs(din, dst, nsteps, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->d.sched = dst;
+  s->d.nsched = nsteps;
   s->has_src = true;
   if (s->gexec) {  // captured graphs hold the old pointers
     (void)hipGraphExecDestroy(s->gexec);
@@ -437,7 +462,7 @@ int teuk_trajectory_schedule(const double *rows, int nrows, int nsteps, double s
 int teuk_step(teuk_solver *s, int nsteps) {
   if (!s || nsteps < 0) return TEUK_EINVAL;
   if (s->k_host + nsteps > s->nt) return TEUK_ERANGE;
-  if (s->has_src && s->k_host + nsteps > s->d.nsteps_src) return TEUK_ERANGE;
+  if (s->has_src && s->k_host + nsteps > s->d.nsched) return TEUK_ERANGE;
   int rc = TEUK_OK;
   // Direct launches of n steps, closed by one finalize.
   auto direct = [&](int n) {

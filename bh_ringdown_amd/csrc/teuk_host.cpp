@@ -177,4 +177,64 @@ void host_coeff_tables(int N, int M, double xmin, double xmax, double dx, double
   }
 }
 
+// kern.cu:2671-2672: the source picks up e^{-i m a/(r+ - r-) ln((r - r+)/(r - r-))},
+// evaluated at the grid radius of each column.
+void host_source_phase(int N, double mass, double a, double m, const double *r_c, double *phase) {
+  const double rpl = mass + std::sqrt(mass * mass - a * a);
+  const double rmi = mass - std::sqrt(mass * mass - a * a);
+  for (int j = 0; j < N; j++) {
+    const double r = r_c[j];
+    double arg = 0.0;
+    if (a != 0.0) arg = -m * a / (rpl - rmi) * std::log((r - rpl) / (r - rmi));
+    phase[j] = std::cos(arg);
+    phase[N + j] = std::sin(arg);
+  }
+}
+
+// kern.cu:1186-1206 and 2673 for one step: the patch origin from the
+// particle's tortoise radius and theta, the radial Gaussian width
+// wr = 2 dx (1 - r+/r_p), the turn-on (1 - e^{-(t - t0)/25})^4, and the
+// transcendental inputs of the T_4 coefficients.  For a = 0 the tortoise
+// radius uses its Schwarzschild form (the reference's Kerr form is 0 * inf
+// there).
+void host_source_prepare(const Grid &g, const SrcStep &st, SrcPrep &out) {
+  const double a = g.a, mass = g.mass;
+  const double rpl = mass + std::sqrt(mass * mass - a * a);
+  const double rmi = mass - std::sqrt(mass * mass - a * a);
+  double xp;
+  if (a != 0.0)
+    xp = st.rp + (rpl * rpl + a * a) / (rpl - rmi) * std::log(st.rp / rpl - 1.0) -
+         (rmi * rmi + a * a) / (rpl - rmi) * std::log(st.rp / rmi - 1.0);
+  else
+    xp = st.rp + 2.0 * mass * std::log(st.rp / (2.0 * mass) - 1.0);
+  const int ip = (int)((xp - g.xmin) / g.dx);
+  const int it = (int)(st.tp / g.dtheta);
+  SrcDev &d = out.dev;
+  for (int i = 0; i < 12; i++) d.c[i] = 0.0;
+  d.rp = st.rp;
+  d.tp = st.tp;
+  d.wr = (2.0 * g.dx) * (1.0 - rpl / st.rp);
+  d.turnon = std::pow(1.0 - std::exp(-(st.timer - st.start) / 25.0), 4);
+  d.ip0 = ip - g.ptsx / 2;
+  d.it0 = it - g.ptsy / 2;
+  teuk_t4::StepIn &in = out.in;
+  in.mass = mass;
+  in.a = a;
+  in.E = st.E;
+  in.lz = st.lz;
+  in.nmu = 1.0;  // kern.cu:1190
+  in.mm = g.m;
+  in.rp = st.rp;
+  in.drdt = st.drdt;
+  in.d2rdt2 = st.d2rdt2;
+  in.dthdt = st.dthdt;
+  in.d2thdt2 = st.d2thdt2;
+  in.dphidt = st.dphidt;
+  in.d2phidt2 = st.d2phidt2;
+  in.stp = std::sin(st.tp);
+  in.ctp = std::cos(st.tp);
+  in.cmphi = std::cos(g.m * st.phip);
+  in.smphi = std::sin(g.m * st.phip);
+}
+
 }  // namespace teuk

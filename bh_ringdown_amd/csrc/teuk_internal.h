@@ -7,6 +7,9 @@
 #include <cstddef>
 #include <cstdint>
 
+#define TEUK_HD __host__ __device__ inline
+#include "teuk_t4.h"
+
 namespace teuk {
 
 // parm.h:12 -- the 15-digit pi the reference uses for dtheta, cot and the
@@ -73,6 +76,21 @@ struct SrcStep {
   double drdt, d2rdt2, dthdt, d2thdt2, dphidt, d2phidt2;
 };
 
+// One step of the source schedule on the device: canonical T_4 coefficients
+// (filled by k_source_coeffs), particle position, radial Gaussian width,
+// turn-on factor and the patch origin (kern.cu:1191-1206, 2673).
+struct SrcDev {
+  double c[12];
+  double rp, tp, wr, turnon;
+  int ip0, it0;
+};
+
+// Host-side preparation of one step (teuk_host.cpp).
+struct SrcPrep {
+  teuk_t4::StepIn in;
+  SrcDev dev;
+};
+
 // Device pointers of one solver.
 struct DevState {
   double *U, *C, *D, *A;   // field states (4 planes each); A only for RK4
@@ -81,16 +99,18 @@ struct DevState {
   double *dbdx;            // [M][N] rho derivative of bb (kern.cu:192-196)
   double *scratch;         // [M][N] set-up scratch (bb plane)
   double *theta, *r_c;     // [M], [N]
+  double *phase;           // [2][N] cos, sin of the source phase (kern.cu:2671-2672)
   double *patch;           // [2][ptsy][ptsx] source patch
   int *patch_org;          // [2] {row0, col0}
-  SrcStep *steps;          // [nsteps] source schedule, or null
-  int nsteps_src;
+  SrcDev *sched;           // [nsched] source schedule, or null
+  int nsched;
   Ext ext;
   Ctl *ctl;
 };
 
 // Launchers (teuk_kernels.hip).
 hipError_t launch_init_coeffs(const Grid &g, const DevState &d, hipStream_t s);
+hipError_t launch_source_coeffs(const teuk_t4::StepIn *in, SrcDev *sched, int n, hipStream_t s);
 hipError_t launch_source(const Grid &g, const DevState &d, int k_rel, hipStream_t s);
 hipError_t launch_stage(const Grid &g, const DevState &d, int integrator, int stage,
                         bool has_src, int k_rel, hipStream_t s);
@@ -106,5 +126,9 @@ void host_projection_weights(int M, double m, double *w /*[7][5*(M/4)][3]*/);
 void host_coeff_tables(int N, int M, double xmin, double xmax, double dx, double mass, double a,
                        double m, const double *r_c, const double *theta, double *ktab,
                        double *rtab);
+// Source phase e^{-i m a ln((r-r+)/(r-r-))/(r+-r-)} per column, [2][N].
+void host_source_phase(int N, double mass, double a, double m, const double *r_c, double *phase);
+// Per-step transcendentals and patch placement of the source (kern.cu:1186-1206, 2673).
+void host_source_prepare(const Grid &g, const SrcStep &st, SrcPrep &out);
 
 }  // namespace teuk

@@ -115,8 +115,12 @@ __global__ __launch_bounds__(256) void k_init_dbdx(Grid g, const double *__restr
 //     STAGE 4: out = acc + dt/6 L(in)
 // The last stage also integrates Psi at the extraction column twice in time
 // (body.cu:177-187) and snapshots it every Fr steps.
+#ifndef TEUK_STAGE_WAVES_PER_EU
+#define TEUK_STAGE_WAVES_PER_EU 2
+#endif
 template <int INTEG, int STAGE>
-__global__ __launch_bounds__(256) void k_stage(Grid g, const double *__restrict__ in, const double *base,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEUK_STAGE_WAVES_PER_EU)))
+void k_stage(Grid g, const double *__restrict__ in, const double *base,
                                                double *out, double *acc, const double *__restrict__ ktab,
                                                const double *__restrict__ rtab,
                                                const double *__restrict__ dbdx,
@@ -239,12 +243,38 @@ __global__ __launch_bounds__(256) void k_stage(Grid g, const double *__restrict_
     }
   };
 
+  // Per-row global inputs, fetched one row ahead: the LDS fences below keep
+  // the compiler from hoisting loads across rows, so the next row's loads
+  // are issued before the current row's fences and overlap its arithmetic.
+  struct RowIn {
+    double pr, pi, halo[4], base[4], dbdx;
+  };
+  const int slot = lane < kHalo ? lane : kWave + lane;  // halo slot of lanes 0..7
+  const int hcol = a0 - kHalo + slot;
+  const bool halo_lane = lane < 2 * kHalo && hcol >= 0 && hcol < N;
+  constexpr bool kBase = (INTEG == 0 && STAGE >= 2) || (INTEG == 1 && (STAGE == 2 || STAGE == 3));
+  auto load_row = [&](int b) {
+    RowIn x;
+    const size_t c0 = (size_t)b * N;
+    x.pr = in[2 * NM + c0 + a];
+    x.pi = in[3 * NM + c0 + a];
+#pragma unroll
+    for (int f = 0; f < 4; f++) x.halo[f] = halo_lane ? in[f * NM + c0 + hcol] : 0.0;
+#pragma unroll
+    for (int f = 0; f < 4; f++) x.base[f] = kBase ? base[f * NM + c0 + a] : 0.0;
+    x.dbdx = dbdx[c0 + a];
+    return x;
+  };
+  RowIn cur = load_row(b0 < 3 ? 3 : b0);
+
 #pragma unroll
   for (int i = 0; i < kRows; i++) {
     const int b = b0 + i;
     if (b < 3 || b >= M - 3) continue;  // uniform over the wavefront
     const size_t c0 = (size_t)b * N;
-    const double pr0 = in[2 * NM + c0 + a], pi0 = in[3 * NM + c0 + a];
+    RowIn nxt = cur;
+    if (i + 1 < kRows && b + 1 < M - 3) nxt = load_row(b + 1);
+    const double pr0 = cur.pr, pi0 = cur.pi;
     const double qr0 = wqr[i + 3], qi0 = wqi[i + 3];
 
     // Stage the row (4 fields, 64 columns + 4-column halos) in LDS.
@@ -252,13 +282,9 @@ __global__ __launch_bounds__(256) void k_stage(Grid g, const double *__restrict_
     L[1 * kLdsCols + lane + kHalo] = qi0;
     L[2 * kLdsCols + lane + kHalo] = pr0;
     L[3 * kLdsCols + lane + kHalo] = pi0;
-    if (lane < 2 * kHalo) {
-      const int slot = lane < kHalo ? lane : kWave + lane;
-      const int col = a0 - kHalo + slot;
-      if (col >= 0 && col < N) {
+    if (halo_lane) {
 #pragma unroll
-        for (int f = 0; f < 4; f++) L[f * kLdsCols + slot] = in[f * NM + c0 + col];
-      }
+      for (int f = 0; f < 4; f++) L[f * kLdsCols + slot] = cur.halo[f];
     }
     wave_sync();
 
@@ -298,7 +324,7 @@ __global__ __launch_bounds__(256) void k_stage(Grid g, const double *__restrict_
 
       const size_t id = c0 + a;
       const Coef cf = coef_point(K, rtab[b], rtab[M + b], rtab[2 * M + b]);
-      const double dre = cf.dre - dbdx[id] * cf.ee;
+      const double dre = cf.dre - cur.dbdx * cf.ee;
 
       double sr = 0.0, si = 0.0;
       const int sb = b - it0, sa = a - ip0;
@@ -322,10 +348,10 @@ __global__ __launch_bounds__(256) void k_stage(Grid g, const double *__restrict_
           for (int f = 0; f < 4; f++) v[f] = u[f] + dt * r[f];
         } else if (STAGE == 2) {
 #pragma unroll
-          for (int f = 0; f < 4; f++) v[f] = 0.25 * (3.0 * base[f * NM + id] + u[f] + dt * r[f]);
+          for (int f = 0; f < 4; f++) v[f] = 0.25 * (3.0 * cur.base[f] + u[f] + dt * r[f]);
         } else {
 #pragma unroll
-          for (int f = 0; f < 4; f++) v[f] = kThird * (base[f * NM + id] + 2.0 * u[f] + 2.0 * dt * r[f]);
+          for (int f = 0; f < 4; f++) v[f] = kThird * (cur.base[f] + 2.0 * u[f] + 2.0 * dt * r[f]);
         }
       } else {
         const double h2 = 0.5 * dt, h6 = dt / 6.0, h3 = dt / 3.0;
@@ -339,7 +365,7 @@ __global__ __launch_bounds__(256) void k_stage(Grid g, const double *__restrict_
           const double h = (STAGE == 2) ? h2 : dt;
 #pragma unroll
           for (int f = 0; f < 4; f++) {
-            v[f] = base[f * NM + id] + h * r[f];
+            v[f] = cur.base[f] + h * r[f];
             acc[f * NM + id] = acc[f * NM + id] + h3 * r[f];
           }
         } else {
@@ -351,6 +377,7 @@ __global__ __launch_bounds__(256) void k_stage(Grid g, const double *__restrict_
     wave_sync();  // LDS row is free for the next iteration
     rho_fix(v);
     store_row(b, v);
+    cur = nxt;
     if (b == 3 || b == M - 5) {
 #pragma unroll
       for (int f = 0; f < 4; f++) lo[f] = v[f];
@@ -473,7 +500,7 @@ __global__ __launch_bounds__(256) void k_finalize(Grid g, Ext ext, Ctl *ctl, int
 
 }  // namespace
 
-bool source_available() { return kSourceAvailable; }
+bool source_available() { return true; }
 
 // ---------------------------------------------------------------------------
 // Launchers.
@@ -529,8 +556,14 @@ hipError_t launch_finalize(const Grid &g, const DevState &d, int n, hipStream_t 
 
 hipError_t launch_source(const Grid &g, const DevState &d, int k_rel, hipStream_t s) {
   const int cells = g.ptsx * g.ptsy;
-  hipLaunchKernelGGL(k_source, dim3((cells + 255) / 256), dim3(256), 0, s, g, d.theta, d.r_c, d.steps,
-                     d.nsteps_src, d.patch, d.patch_org, d.ctl, k_rel);
+  hipLaunchKernelGGL(k_source, dim3((cells + 255) / 256), dim3(256), 0, s, g, d.r_c, d.rtab, d.theta, d.phase,
+                     d.sched, d.nsched, d.patch, d.patch_org, d.ctl, k_rel);
+  return hipGetLastError();
+}
+
+hipError_t launch_source_coeffs(const teuk_t4::StepIn *in, SrcDev *sched, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_source_coeffs, dim3((n + 63) / 64), dim3(64), 0, s, in, sched, n);
   return hipGetLastError();
 }
 

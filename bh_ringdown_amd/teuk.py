@@ -13,7 +13,8 @@ import os
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "lib", "libteuk.so")
+# TEUK_LIB selects another build of the same library (e.g. a tuning variant).
+LIB_PATH = os.environ.get("TEUK_LIB") or os.path.join(PKG, "lib", "libteuk.so")
 PI15 = 3.14159265358979  # parm.h:12
 
 _dp = ctypes.POINTER(ctypes.c_double)

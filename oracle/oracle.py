@@ -67,6 +67,9 @@ def _load(name):
     lib.orc_project.restype = ctypes.c_double
     lib.orc_run.argtypes = [P, _dp, _dp, _dp, ctypes.c_int, ctypes.POINTER(SrcStep), _dp, _dp, _dp]
     lib.orc_run.restype = ctypes.c_int
+    lib.orc_t4_coeffs.argtypes = [_dp, _dp]
+    lib.orc_exp_neg.argtypes = [ctypes.c_double]
+    lib.orc_exp_neg.restype = ctypes.c_double
     return lib
 
 
@@ -124,6 +127,23 @@ def step(p, coef, theta, u, src=None, work=None):
 def source(p, theta, r_c, st):
     out = np.zeros((2, p.M, p.N))
     lib().orc_source(ctypes.byref(p), _ptr(theta), _ptr(r_c), ctypes.byref(st), _ptr(out))
+    return out
+
+
+T4_IN = ["mass", "a", "E", "lz", "nmu", "mm", "rp", "drdt", "d2rdt2", "dthdt", "d2thdt2",
+         "dphidt", "d2phidt2", "stp", "ctp", "cmphi", "smphi"]
+
+
+def t4_coeffs(state):
+    """Canonical T_4 coefficients x e^{-i m phi}/pi for a particle state dict
+    (keys of T4_IN, with tp and phip instead of the trigonometric values)."""
+    import math
+    v = dict(state)
+    v["stp"], v["ctp"] = math.sin(v["tp"]), math.cos(v["tp"])
+    v["cmphi"], v["smphi"] = math.cos(v["mm"] * v["phip"]), math.sin(v["mm"] * v["phip"])
+    inp = np.array([float(v[k]) for k in T4_IN])
+    out = np.zeros(12)
+    lib().orc_t4_coeffs(_ptr(inp), _ptr(out))
     return out
 
 

@@ -64,7 +64,7 @@ EXPORTS = [
     "teuk_grid", "teuk_default_params", "teuk_create", "teuk_destroy", "teuk_set_fields",
     "teuk_get_fields", "teuk_set_source", "teuk_set_trajectory", "teuk_step", "teuk_sync",
     "teuk_progress", "teuk_read_output", "teuk_read_psi", "teuk_enable_timing", "teuk_timing",
-    "teuk_footprint", "teuk_body", "teuk_strerror", "teuk_trajectory_schedule",
+    "teuk_footprint", "teuk_body", "teuk_strerror", "teuk_trajectory_schedule", "teuk_traj_plunge",
 ]
 
 _lib = None
@@ -106,6 +106,8 @@ def lib():
         "teuk_strerror": ([ctypes.c_int], ctypes.c_char_p),
         "teuk_trajectory_schedule": ([_dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                       ctypes.POINTER(SourceStep)], ctypes.c_int),
+        "teuk_traj_plunge": ([ctypes.c_double] * 6 + [ctypes.c_ulonglong, _dp, ctypes.c_int,
+                                                      ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(l, name)
@@ -155,6 +157,21 @@ def trajectory_schedule(rows, nsteps, dt, start=0.0):
     _check(lib().teuk_trajectory_schedule(_ptr(rows), rows.shape[0], nsteps, start, dt, out),
            "teuk_trajectory_schedule")
     return list(out)
+
+
+def plunge_trajectory(a, eta, iota, dt_out, t_final, seed=0, r0_over_isco=1.02):
+    """Seeded transition-plus-plunge worldline as (n, 10) .traj rows."""
+    cap = int(t_final / dt_out) + 4
+    rows = np.zeros((cap, 10))
+    n = ctypes.c_int()
+    _check(lib().teuk_traj_plunge(a, eta, iota, r0_over_isco, dt_out, t_final, seed, _ptr(rows), cap,
+                                  ctypes.byref(n)), "teuk_traj_plunge")
+    return rows[: n.value].copy()
+
+
+def write_traj(path, rows):
+    """Write rows in the .traj text format (numpy's %.18e, as the reference files)."""
+    np.savetxt(path, rows, fmt="%.18e")
 
 
 def footprint(p, nsteps=None):

@@ -143,6 +143,18 @@ int teuk_body(const teuk_params *p, const double *r_c, const double *theta,
               const char *traj_path, const char *out_path, const char *full_path,
               int verbose);
 
+/* ---- trajectory source ----------------------------------------------------- */
+/* Seeded transition-plus-plunge worldline (mass ratio eta, inclination iota
+ * in radians, start at r0 = r0_over_isco x the inclined ISCO radius), sampled
+ * every dt_out of Boyer-Lindquist time up to t_final, as .traj rows
+ * (10 doubles each: t r theta dr/dt dtheta/dt dphi/dt phi E Lz Q, the format
+ * GenOT/WL_BL_RK.cc:214-221 writes and body.cu:97 reads).  Use
+ * dt_out = dt/2 for a solver step dt (body.cu:112-113 reads two rows per
+ * step).  Replaces running GenOT, whose adiabatic input files are not
+ * part of the reference tree. */
+int teuk_traj_plunge(double a, double eta, double iota, double r0_over_isco, double dt_out,
+                     double t_final, unsigned long long seed, double *rows, int cap, int *nrows);
+
 /* Human-readable text for an error code. */
 const char *teuk_strerror(int code);
 

@@ -68,8 +68,9 @@ def test_patch_at_domain_edges():
     """Particle close to the inner rho edge and the theta axis: the patch is
     clipped by the reference's guards (index > 0, row < M) on both sides."""
     steps = 60
-    p = teuk.default_params(512, 32, -20.0, 80.0, 0.6, 2, dt_over_dx=0.5)
+    p = teuk.default_params(512, 32, -10.0, 90.0, 0.6, 2, dt_over_dx=0.5)
     p.T = steps * p.dt * (1 + 1e-12)
-    rows = inspiral_rows(2 * steps + 4, p.dt, r0=3.0, rdot=0.0, th0=0.2, thamp=0.0)
+    # r = 1.85 (r+ = 1.8): tortoise radius about -6.3, patch origin below column 0
+    rows = inspiral_rows(2 * steps + 4, p.dt, r0=1.85, rdot=0.0, th0=0.2, thamp=0.0)
     ug, uo, *_ = run_both(p, steps, rows)
     assert np.array_equal(ug, uo)

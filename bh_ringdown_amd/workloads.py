@@ -52,3 +52,11 @@ def single_16384x64(T=1500.0, integrator=0, Fr=2):
     p.integrator = integrator
     p.j_extract = j_at(p, 1000.0)
     return p
+
+
+def plunge_rows(p, nsteps, eta=1e-3, iota_deg=30.0, seed=0, r0_over_isco=1.02):
+    """Seeded transition-plus-plunge .traj rows for spin p.a, sampled at dt/2
+    and long enough for nsteps of the reference's two-rows-per-step reading."""
+    t_final = (2 * nsteps + 4) * (p.dt / 2)
+    return teuk.plunge_trajectory(p.a, eta, math.radians(iota_deg), p.dt / 2, t_final, seed=seed,
+                                  r0_over_isco=r0_over_isco)

@@ -2,9 +2,10 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-A step is one full SSP-RK3 time step of the single solve: the particle
-source deposit on its 128x128 patch, three fused stage kernels (the last
-one also extracts Psi and integrates h) over N=16384 rho points on
+A step is one full classic-RK4 time step of the single solve (--integrator
+rk3: the reference's SSP-RK3): the particle source deposit on its 128x128
+patch, then one fused kernel per RK stage (the last one also extracts Psi
+and integrates h) over N=16384 rho points on
 [-400, 1200] M and M=64 theta points, a=0.9, m=2, dt = 0.5 drho, zero
 initial fields, source on a seeded plunge (mass ratio 1e-3, inclination
 30 degrees, from 1.02 x the ISCO).  One point-update advances (Psi, Pi) at
@@ -25,7 +26,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
-B_ALG_RK3 = 256.0              # algorithmic bytes per point-update, SSP-RK3
+# Algorithmic bytes per point-update (state traffic of one sweep per stage):
+# SSP-RK3 64 + 96 + 96; classic RK4 in accumulator form 96 + 160 + 160 + 96.
+B_ALG = {"rk3": 256.0, "rk4": 512.0}
+STAGES = {"rk3": 3, "rk4": 4}
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_stage_pmc_16384x64.txt")
 
 
@@ -43,7 +47,7 @@ def pmc_traffic():
 
 def cpu_baseline(p, rows, seconds=12.0):
     """The oracle (g++ -O3 OpenMP build) on the first steps of the same
-    workload: source, three RK stages, boundaries, extraction and projection
+    workload: source, the RK stages, boundaries, extraction and projection
     per step.  The one-off coefficient set-up is timed separately and
     subtracted (a one-step run)."""
     import numpy as np
@@ -51,7 +55,9 @@ def cpu_baseline(p, rows, seconds=12.0):
     from oracle import oracle as O
     nth = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
     _, r_c, th = teuk.grid(p)
-    op = O.make_params(p.N, p.M, p.xmin, p.xmax, p.a, p.m, dt=p.dt, j_extract=p.j_extract, nthreads=nth)
+    op = O.make_params(p.N, p.M, p.xmin, p.xmax, p.a, p.m, dt=p.dt, j_extract=p.j_extract, nthreads=nth,
+                       integrator=p.integrator)
+    name = "RK4" if p.integrator == 1 else "SSP-RK3"
 
     def timed(n):
         sched = teuk.trajectory_schedule(rows, n, p.dt)
@@ -69,7 +75,7 @@ def cpu_baseline(p, rows, seconds=12.0):
     n = int(max(4, min(2000, seconds / per)))
     el = timed(n + 1) - t1
     return {"value": p.N * p.M * n / el, "unit": "point-updates/s", "cores": nth, "kind": "port",
-            "sample": f"first {n} SSP-RK3 steps (with source, extraction, projection) of the "
+            "sample": f"first {n} {name} steps (with source, extraction, projection) of the "
                       f"{p.N}x{p.M} plunge solve; oracle g++ -O3 OpenMP, set-up excluded"}
 
 
@@ -80,6 +86,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--timing-steps", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--integrator", choices=["rk4", "rk3"], default="rk4")
+    ap.add_argument("--wpb", type=int, default=0, help="wavefronts per block (0 = default)")
+    ap.add_argument("--wpe", type=int, default=0, help="occupancy target, waves per SIMD (0 = default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -87,7 +96,8 @@ def main():
         return 2
 
     from bh_ringdown_amd import teuk, workloads
-    p = workloads.single_16384x64()
+    integ = 1 if args.integrator == "rk4" else 0
+    p = workloads.single_16384x64(integrator=integ)
     need = args.warmup + args.steps + args.timing_steps
     p.T = max(p.T, need * p.dt * (1.0 + 1e-12))
     nt = int(p.T / p.dt)
@@ -96,6 +106,8 @@ def main():
     pts = p.N * p.M
     log(f"bench: workload single_16384x64_plunge N={p.N} M={p.M} warmup={args.warmup} steps={args.steps}")
     with teuk.Solver(p, r_c, th) as s:
+        if args.wpb or args.wpe:
+            s.set_tiling(args.wpb or 4, args.wpe or 2)
         s.set_trajectory(rows, nt)
         s.step(args.warmup)
         s.sync()
@@ -111,8 +123,9 @@ def main():
         launches, ms = s.timing()
     value = pts * args.steps / el
     per_launch_s = ms / 1e3 / launches
-    # a step is 3 stage launches; B_alg is per full step
-    achieved = (B_ALG_RK3 / 3.0) * pts / per_launch_s / 1e9
+    # B_alg is per full step; a step is STAGES stage launches
+    b_alg = B_ALG[args.integrator]
+    achieved = (b_alg / STAGES[args.integrator]) * pts / per_launch_s / 1e9
     traffic = pmc_traffic()
     out = {
         "metric": "FP64 point-updates/s (%HBM roofline)",
@@ -128,12 +141,14 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "single_16384x64_plunge", "N": p.N, "M": p.M, "rho": [p.xmin, p.xmax],
-                   "a": p.a, "m": p.m, "dt_over_drho": 0.5, "integrator": "ssp-rk3", "t_final": p.T,
+                   "a": p.a, "m": p.m, "dt_over_drho": 0.5,
+                   "integrator": {"rk4": "classic RK4", "rk3": "SSP-RK3"}[args.integrator],
+                   "t_final": p.T,
                    "source": "seeded plunge, mass ratio 1e-3, inclination 30 deg, from 1.02 r_isco",
                    "extraction_rho": p.xmin + p.j_extract * p.dx},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_stage (fused RK stage)", "bytes_per_point_update": B_ALG_RK3,
+                     "kernel": "k_stage (fused RK stage)", "bytes_per_point_update": b_alg,
                      "avg_launch_us": per_launch_s * 1e6},
     }
     if not args.no_cpu:

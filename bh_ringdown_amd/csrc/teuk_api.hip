@@ -15,10 +15,14 @@
 
 using namespace teuk;
 
+struct teuk_batch;
+
 struct teuk_solver {
   teuk_params P;
   Grid g;
   DevState d;
+  Tiling tiling;
+  teuk_batch *batch = nullptr;   // set while a batch drives this solver
   hipStream_t stream = nullptr;
   int nt = 0;              // scheduled steps: (int)(T/dt), body.cu:59
   int k_host = 0;          // steps enqueued
@@ -35,7 +39,33 @@ struct teuk_solver {
   std::vector<void *> allocs;
 };
 
+// A set of solvers stepped together by the batched kernels (one launch per
+// stage for every member), with graph-captured chunks of steps.
+struct teuk_batch {
+  int integrator = 0;
+  Tiling tiling{4, 2};
+  int chunk = 32;          // steps per captured graph
+  int order = 0;           // tile table: 0 = contiguous run per XCD, 1 = dealt in list order
+  long long budget = 0;    // bytes of member footprint allowed (0 = no limit)
+  long long used = 0;
+  std::vector<teuk_solver *> members;
+  std::vector<long long> member_bytes;
+  hipStream_t stream = nullptr;
+  SolveDesc *d_desc = nullptr;
+  int desc_cap = 0;
+  int2 *d_tiles = nullptr;
+  int tiles_cap = 0;
+  int ntiles = 0;
+  int max_cells = 0;
+  bool dirty = true;
+  hipGraphExec_t gexec = nullptr;
+  int graph_steps = 0;
+};
+
 namespace {
+
+// The stream that runs a solver's steps: its own, or its batch's.
+hipStream_t run_stream(const teuk_solver *s) { return s->batch ? s->batch->stream : s->stream; }
 
 #define HIPCHK(x)                              \
   do {                                         \
@@ -103,11 +133,11 @@ int enqueue_step(teuk_solver *s, int k_rel) {
         }
       }
       HIPCHK(hipEventRecord(s->ev[s->ev_used], s->stream));
-      HIPCHK(launch_stage(s->g, s->d, s->P.integrator, st, s->has_src, k_rel, s->stream));
+      HIPCHK(launch_stage(s->g, s->d, s->P.integrator, st, s->has_src, k_rel, s->tiling, s->stream));
       HIPCHK(hipEventRecord(s->ev[s->ev_used + 1], s->stream));
       s->ev_used += 2;
     } else {
-      HIPCHK(launch_stage(s->g, s->d, s->P.integrator, st, s->has_src, k_rel, s->stream));
+      HIPCHK(launch_stage(s->g, s->d, s->P.integrator, st, s->has_src, k_rel, s->tiling, s->stream));
     }
   }
   return TEUK_OK;
@@ -250,7 +280,7 @@ long long teuk_footprint(const teuk_params *p, int nsteps) {
   bytes += 8 * (2LL * p->ptsx * p->ptsy) + 8;
   bytes += 8 * (4LL * p->M + 2 * p->M + 1 + 7LL * 5 * (p->M / 4) * 3);
   bytes += nrec * (16 * 8 + 2 * 4 + 6LL * p->M * 8);
-  bytes += (long long)nsteps * (long long)sizeof(SrcStep);
+  bytes += (long long)nsteps * (long long)(sizeof(SrcDev) + sizeof(teuk_t4::StepIn));
   return bytes;
 }
 
@@ -262,6 +292,7 @@ int teuk_create(const teuk_params *p, const double *r_c, const double *theta, te
   teuk_solver *s = new teuk_solver();
   s->P = *p;
   s->nt = scheduled_steps(p);
+  s->tiling = default_tiling(p->M);
   Grid &g = s->g;
   g.N = p->N;
   g.M = p->M;
@@ -376,6 +407,7 @@ int teuk_create(const teuk_params *p, const double *r_c, const double *theta, te
 
 void teuk_destroy(teuk_solver *s) {
   if (!s) return;
+  if (s->batch) (void)teuk_batch_remove(s->batch, s);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
   for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
@@ -387,21 +419,22 @@ void teuk_destroy(teuk_solver *s) {
 int teuk_set_fields(teuk_solver *s, const double *u) {
   if (!s || !u) return TEUK_EINVAL;
   const size_t bytes = 4 * (size_t)s->P.N * s->P.M * sizeof(double);
-  HIPCHK(hipMemcpyAsync(s->d.U, u, bytes, hipMemcpyHostToDevice, s->stream));
-  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipMemcpyAsync(s->d.U, u, bytes, hipMemcpyHostToDevice, run_stream(s)));
+  HIPCHK(hipStreamSynchronize(run_stream(s)));
   return TEUK_OK;
 }
 
 int teuk_get_fields(teuk_solver *s, double *u) {
   if (!s || !u) return TEUK_EINVAL;
   const size_t bytes = 4 * (size_t)s->P.N * s->P.M * sizeof(double);
-  HIPCHK(hipMemcpyAsync(u, s->d.U, bytes, hipMemcpyDeviceToHost, s->stream));
-  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipMemcpyAsync(u, s->d.U, bytes, hipMemcpyDeviceToHost, run_stream(s)));
+  HIPCHK(hipStreamSynchronize(run_stream(s)));
   return TEUK_OK;
 }
 
 int teuk_set_source(teuk_solver *s, const teuk_source_step *steps, int nsteps) {
   if (!s || nsteps < 0) return TEUK_EINVAL;
+  if (s->batch) return TEUK_EINVAL;  // the batch's descriptors hold the schedule
   if (!steps || nsteps == 0) {
     s->has_src = false;
     return TEUK_OK;
@@ -461,6 +494,7 @@ int teuk_trajectory_schedule(const double *rows, int nrows, int nsteps, double s
 
 int teuk_step(teuk_solver *s, int nsteps) {
   if (!s || nsteps < 0) return TEUK_EINVAL;
+  if (s->batch) return TEUK_EINVAL;  // a batch drives this solver
   if (s->k_host + nsteps > s->nt) return TEUK_ERANGE;
   if (s->has_src && s->k_host + nsteps > s->d.nsched) return TEUK_ERANGE;
   int rc = TEUK_OK;
@@ -491,7 +525,7 @@ int teuk_step(teuk_solver *s, int nsteps) {
 
 int teuk_sync(teuk_solver *s) {
   if (!s) return TEUK_EINVAL;
-  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipStreamSynchronize(run_stream(s)));
   if (s->timing) flush_timing(s);
   Ctl c;
   HIPCHK(hipMemcpy(&c, s->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
@@ -501,7 +535,7 @@ int teuk_sync(teuk_solver *s) {
 int teuk_progress(teuk_solver *s, int *k, double *t) {
   if (!s) return TEUK_EINVAL;
   Ctl c;
-  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipStreamSynchronize(run_stream(s)));
   HIPCHK(hipMemcpy(&c, s->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
   if (k) *k = c.k;
   if (t) *t = c.timer;
@@ -511,7 +545,7 @@ int teuk_progress(teuk_solver *s, int *k, double *t) {
 int teuk_read_output(teuk_solver *s, int first, int max, teuk_output_row *rows, int *n) {
   if (!s || !rows || first < 0 || max < 0) return TEUK_EINVAL;
   Ctl c;
-  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipStreamSynchronize(run_stream(s)));
   HIPCHK(hipMemcpy(&c, s->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
   const int cnt = std::max(0, std::min(max, c.nrec - first));
   if (cnt > 0) {
@@ -534,7 +568,7 @@ int teuk_read_output(teuk_solver *s, int first, int max, teuk_output_row *rows, 
 int teuk_read_psi(teuk_solver *s, int first, int max, double *psi, int *n) {
   if (!s || !psi || first < 0 || max < 0) return TEUK_EINVAL;
   Ctl c;
-  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipStreamSynchronize(run_stream(s)));
   HIPCHK(hipMemcpy(&c, s->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
   const int cnt = std::max(0, std::min(max, c.nrec - first));
   // Each snapshot holds Psi (re, im) followed by the two time integrals.
@@ -637,3 +671,262 @@ int teuk_body(const teuk_params *p, const double *r_c, const double *theta, cons
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Launch shape of one solver's stage kernels.
+int teuk_set_tiling(teuk_solver *s, int waves_per_block, int waves_per_eu) {
+  if (!s || s->batch) return TEUK_EINVAL;
+  if (!(waves_per_block == 1 || waves_per_block == 2 || waves_per_block == 4)) return TEUK_EINVAL;
+  if (!(waves_per_eu == 2 || waves_per_eu == 4)) return TEUK_EINVAL;
+  if (s->P.M % (kRows * waves_per_block) != 0) return TEUK_EINVAL;
+  s->tiling.wpb = waves_per_block;
+  s->tiling.wpe = waves_per_eu;
+  if (s->gexec) {
+    HIPCHK(hipStreamSynchronize(s->stream));
+    (void)hipGraphExecDestroy(s->gexec);
+    s->gexec = nullptr;
+  }
+  return TEUK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batches.
+namespace {
+
+int member_steps(const teuk_solver *s) { return s->has_src ? std::min(s->nt, s->d.nsched) : s->nt; }
+
+void batch_drop_graph(teuk_batch *b) {
+  if (b->gexec) {
+    (void)hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+  }
+}
+
+// Descriptors and the tile table of the members that still have steps.
+int batch_rebuild(teuk_batch *b) {
+  HIPCHK(hipStreamSynchronize(b->stream));  // kernels may still read the old tables
+  batch_drop_graph(b);
+  const int n = (int)b->members.size();
+  std::vector<SolveDesc> hd((size_t)n);
+  std::vector<int2> list;
+  b->max_cells = 0;
+  for (int i = 0; i < n; i++) {
+    teuk_solver *s = b->members[i];
+    SolveDesc &sd = hd[i];
+    std::memset(&sd, 0, sizeof sd);
+    sd.g = s->g;
+    sd.d = s->d;
+    sd.has_src = s->has_src ? 1 : 0;
+    sd.ngroups = s->P.M / (kRows * b->tiling.wpb);
+    sd.nt = member_steps(s);
+    sd.tiles = (s->P.N / kWave) * sd.ngroups;
+    if (s->k_host >= sd.nt) continue;
+    if (s->has_src) b->max_cells = std::max(b->max_cells, s->P.ptsx * s->P.ptsy);
+    for (int t = 0; t < sd.tiles; t++) list.push_back(make_int2(i, t));
+  }
+  std::vector<int2> table;
+  if (b->order == 0) {
+    // Blocks are dealt round-robin over the 8 XCDs: block i lands on XCD
+    // i % 8.  Give XCD x the contiguous run x*per .. (x+1)*per of the list,
+    // so neighbouring tiles of one solve share an L2; pad with no-op tiles.
+    const size_t T = list.size(), per = (T + 7) / 8;
+    table.assign(per * 8, make_int2(-1, 0));
+    for (size_t blk = 0; blk < per * 8; blk++) {
+      const size_t src = (blk & 7) * per + (blk >> 3);
+      if (src < T) table[blk] = list[src];
+    }
+  } else {
+    table = list;
+  }
+  if (n > b->desc_cap) {
+    if (b->d_desc) (void)hipFree(b->d_desc);
+    b->d_desc = nullptr;
+    b->desc_cap = 0;
+    if (hipMalloc(&b->d_desc, (size_t)std::max(n, 1) * sizeof(SolveDesc)) != hipSuccess) return TEUK_ENOMEM;
+    b->desc_cap = std::max(n, 1);
+  }
+  if ((int)table.size() > b->tiles_cap) {
+    if (b->d_tiles) (void)hipFree(b->d_tiles);
+    b->d_tiles = nullptr;
+    b->tiles_cap = 0;
+    if (hipMalloc(&b->d_tiles, std::max<size_t>(table.size(), 1) * sizeof(int2)) != hipSuccess) return TEUK_ENOMEM;
+    b->tiles_cap = (int)std::max<size_t>(table.size(), 1);
+  }
+  if (n > 0) HIPCHK(hipMemcpy(b->d_desc, hd.data(), (size_t)n * sizeof(SolveDesc), hipMemcpyHostToDevice));
+  if (!table.empty())
+    HIPCHK(hipMemcpy(b->d_tiles, table.data(), table.size() * sizeof(int2), hipMemcpyHostToDevice));
+  b->ntiles = (int)table.size();
+  b->dirty = false;
+  return TEUK_OK;
+}
+
+int batch_enqueue_step(teuk_batch *b, int k_rel) {
+  const int n = (int)b->members.size();
+  if (b->max_cells > 0) HIPCHK(launch_source_batch(b->d_desc, n, b->max_cells, k_rel, b->stream));
+  const int nstage = b->integrator == 0 ? 3 : 4;
+  for (int st = 1; st <= nstage; st++)
+    HIPCHK(launch_stage_batch(b->d_desc, b->d_tiles, b->ntiles, b->integrator, st, k_rel, b->tiling, b->stream));
+  return TEUK_OK;
+}
+
+int batch_enqueue(teuk_batch *b, int nsteps) {
+  for (int i = 0; i < nsteps; i++) {
+    int rc = batch_enqueue_step(b, i);
+    if (rc) return rc;
+  }
+  HIPCHK(launch_finalize_batch(b->d_desc, (int)b->members.size(), nsteps, b->stream));
+  return TEUK_OK;
+}
+
+int batch_build_graph(teuk_batch *b) {
+  batch_drop_graph(b);
+  hipGraph_t graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = batch_enqueue(b, b->chunk);
+  hipError_t e = hipStreamEndCapture(b->stream, &graph);
+  if (rc != TEUK_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return TEUK_EHIP;
+  e = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return TEUK_EHIP;
+  b->graph_steps = b->chunk;
+  return TEUK_OK;
+}
+
+}  // namespace
+
+int teuk_batch_create(int integrator, long long hbm_budget, teuk_batch **out) {
+  if (!out || (integrator != 0 && integrator != 1) || hbm_budget < 0) return TEUK_EINVAL;
+  *out = nullptr;
+  teuk_batch *b = new teuk_batch();
+  b->integrator = integrator;
+  b->budget = hbm_budget;
+  b->tiling = default_tiling(32);
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return TEUK_EHIP;
+  }
+  *out = b;
+  return TEUK_OK;
+}
+
+void teuk_batch_destroy(teuk_batch *b) {
+  if (!b) return;
+  (void)hipStreamSynchronize(b->stream);
+  for (teuk_solver *s : b->members) s->batch = nullptr;
+  batch_drop_graph(b);
+  if (b->d_desc) (void)hipFree(b->d_desc);
+  if (b->d_tiles) (void)hipFree(b->d_tiles);
+  (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int teuk_batch_set_tiling(teuk_batch *b, int waves_per_block, int waves_per_eu, int order, int chunk) {
+  if (!b) return TEUK_EINVAL;
+  if (!(waves_per_block == 1 || waves_per_block == 2 || waves_per_block == 4)) return TEUK_EINVAL;
+  if (!(waves_per_eu == 2 || waves_per_eu == 4) || (order != 0 && order != 1)) return TEUK_EINVAL;
+  if (chunk < 1 || chunk > 256) return TEUK_EINVAL;
+  for (teuk_solver *s : b->members)
+    if (s->P.M % (kRows * waves_per_block) != 0) return TEUK_EINVAL;
+  b->tiling.wpb = waves_per_block;
+  b->tiling.wpe = waves_per_eu;
+  b->order = order;
+  b->chunk = chunk;
+  b->dirty = true;
+  return TEUK_OK;
+}
+
+int teuk_batch_add(teuk_batch *b, teuk_solver *s) {
+  if (!b || !s || s->batch) return TEUK_EINVAL;
+  if (s->P.integrator != b->integrator) return TEUK_EINVAL;
+  if (s->P.M % (kRows * b->tiling.wpb) != 0) return TEUK_EINVAL;
+  const long long bytes = teuk_footprint(&s->P, member_steps(s));
+  if (b->budget > 0 && b->used + bytes > b->budget) return TEUK_ENOMEM;
+  HIPCHK(hipStreamSynchronize(s->stream));  // set-up work of the solver's own stream
+  s->batch = b;
+  b->members.push_back(s);
+  b->member_bytes.push_back(bytes);
+  b->used += bytes;
+  b->dirty = true;
+  return TEUK_OK;
+}
+
+int teuk_batch_remove(teuk_batch *b, teuk_solver *s) {
+  if (!b || !s || s->batch != b) return TEUK_EINVAL;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  for (size_t i = 0; i < b->members.size(); i++) {
+    if (b->members[i] == s) {
+      b->used -= b->member_bytes[i];
+      b->members.erase(b->members.begin() + i);
+      b->member_bytes.erase(b->member_bytes.begin() + i);
+      break;
+    }
+  }
+  s->batch = nullptr;
+  b->dirty = true;
+  return TEUK_OK;
+}
+
+int teuk_batch_step(teuk_batch *b, int nsteps) {
+  if (!b || nsteps < 0) return TEUK_EINVAL;
+  int left = nsteps;
+  while (left > 0) {
+    bool any = false;
+    for (teuk_solver *s : b->members) any = any || s->k_host < member_steps(s);
+    if (!any) break;
+    if (b->dirty) {
+      int rc = batch_rebuild(b);
+      if (rc) return rc;
+    }
+    const int n = std::min(left, b->chunk);
+    if (n == b->chunk) {
+      if (!b->gexec || b->graph_steps != b->chunk) {
+        int rc = batch_build_graph(b);
+        if (rc) return rc;
+      }
+      HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+    } else {
+      int rc = batch_enqueue(b, n);
+      if (rc) return rc;
+    }
+    for (teuk_solver *s : b->members) {
+      const int nt = member_steps(s);
+      const int before = s->k_host;
+      s->k_host = std::min(nt, s->k_host + n);
+      if (before < nt && s->k_host >= nt) b->dirty = true;  // drop its tiles next chunk
+    }
+    left -= n;
+  }
+  return TEUK_OK;
+}
+
+int teuk_batch_sync(teuk_batch *b) {
+  if (!b) return TEUK_EINVAL;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  int rc = TEUK_OK;
+  for (teuk_solver *s : b->members) {
+    Ctl c;
+    HIPCHK(hipMemcpy(&c, s->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    if (c.status) rc = TEUK_EDIVERGED;
+  }
+  return rc;
+}
+
+int teuk_batch_info(teuk_batch *b, int *members, int *active, long long *bytes_used) {
+  if (!b) return TEUK_EINVAL;
+  int act = 0;
+  for (teuk_solver *s : b->members) act += s->k_host < member_steps(s) ? 1 : 0;
+  if (members) *members = (int)b->members.size();
+  if (active) *active = act;
+  if (bytes_used) *bytes_used = b->used;
+  return TEUK_OK;
+}
+
+int teuk_remaining(teuk_solver *s, int *steps) {
+  if (!s || !steps) return TEUK_EINVAL;
+  *steps = member_steps(s) - s->k_host;
+  return TEUK_OK;
+}

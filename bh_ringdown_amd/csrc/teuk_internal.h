@@ -108,13 +108,39 @@ struct DevState {
   Ctl *ctl;
 };
 
+// One member of a batch as the batched kernels see it: the run description,
+// the solver's device pointers, its scheduled step count (steps at or past
+// nt are skipped) and its tile count for the batch's block shape.
+struct SolveDesc {
+  Grid g;
+  DevState d;
+  int has_src;
+  int ngroups;         // theta block-rows per rho strip
+  int nt;
+  int tiles;           // (N / 64) * ngroups
+};
+
+// Stage-kernel launch shape: wavefronts per block (1, 2 or 4) and the
+// occupancy hint the kernel was compiled for (waves per SIMD: 2 or 4).
+struct Tiling {
+  int wpb;
+  int wpe;
+};
+
 // Launchers (teuk_kernels.hip).
 hipError_t launch_init_coeffs(const Grid &g, const DevState &d, hipStream_t s);
 hipError_t launch_source_coeffs(const teuk_t4::StepIn *in, SrcDev *sched, int n, hipStream_t s);
 hipError_t launch_source(const Grid &g, const DevState &d, int k_rel, hipStream_t s);
 hipError_t launch_stage(const Grid &g, const DevState &d, int integrator, int stage,
-                        bool has_src, int k_rel, hipStream_t s);
+                        bool has_src, int k_rel, Tiling t, hipStream_t s);
 hipError_t launch_finalize(const Grid &g, const DevState &d, int n, hipStream_t s);
+// Batched forms: one launch covers every member of the batch.
+hipError_t launch_source_batch(const SolveDesc *descs, int nsolves, int max_cells, int k_rel, hipStream_t s);
+hipError_t launch_stage_batch(const SolveDesc *descs, const int2 *tiles, int ntiles, int integrator,
+                              int stage, int k_rel, Tiling t, hipStream_t s);
+hipError_t launch_finalize_batch(const SolveDesc *descs, int nsolves, int n, hipStream_t s);
+// Default launch shape for an M-row grid.
+Tiling default_tiling(int M);
 bool source_available();
 
 // Host helpers (teuk_host.cpp).

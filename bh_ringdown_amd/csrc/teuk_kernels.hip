@@ -118,31 +118,55 @@ __global__ __launch_bounds__(256) void k_init_dbdx(Grid g, const double *__restr
 #ifndef TEUK_STAGE_WAVES_PER_EU
 #define TEUK_STAGE_WAVES_PER_EU 2
 #endif
+
+// Buffers of one stage (the ping-pong between the state planes).
+struct StageIO {
+  const double *in;
+  const double *base;
+  double *out;
+  double *acc;
+};
+
 template <int INTEG, int STAGE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TEUK_STAGE_WAVES_PER_EU)))
-void k_stage(Grid g, const double *__restrict__ in, const double *base,
-                                               double *out, double *acc, const double *__restrict__ ktab,
-                                               const double *__restrict__ rtab,
-                                               const double *__restrict__ dbdx,
-                                               const double *__restrict__ patch,
-                                               const int *__restrict__ porg, int has_src, int ngroups,
-                                               Ext ext, Ctl *ctl, int k_rel) {
+__device__ __forceinline__ StageIO stage_io(const DevState &d) {
+  if (INTEG == 0) {
+    if (STAGE == 1) return {d.U, d.U, d.C, nullptr};
+    if (STAGE == 2) return {d.C, d.U, d.D, nullptr};
+    return {d.D, d.U, d.U, nullptr};
+  }
+  if (STAGE == 1) return {d.U, d.U, d.C, d.A};
+  if (STAGE == 2) return {d.C, d.U, d.D, d.A};
+  if (STAGE == 3) return {d.D, d.U, d.C, d.A};
+  return {d.C, d.U, d.U, d.A};
+}
+
+// One tile of one stage: the wavefront threadIdx.y of block-row `grp` of
+// rho strip `strip` (64 columns), wpb wavefronts per block.
+template <int INTEG, int STAGE>
+__device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int has_src, int k_rel, int strip,
+                                           int grp, int wpb) {
   constexpr bool kLast = (INTEG == 0 && STAGE == 3) || (INTEG == 1 && STAGE == 4);
   __shared__ double lds[4][4][kLdsCols];
   const int N = g.N, M = g.M;
   const size_t NM = (size_t)N * M;
   const int lane = threadIdx.x, w = threadIdx.y;
+  const StageIO io = stage_io<INTEG, STAGE>(d);
+  const double *__restrict__ in = io.in;
+  const double *base = io.base;
+  double *out = io.out;
+  double *acc = io.acc;
+  const double *__restrict__ ktab = d.ktab;
+  const double *__restrict__ rtab = d.rtab;
+  const double *__restrict__ dbdx = d.dbdx;
+  const double *__restrict__ patch = d.patch;
+  const int *__restrict__ porg = d.patch_org;
+  const Ext &ext = d.ext;
+  Ctl *ctl = d.ctl;
 
-  // XCD-aware tile order: the hardware deals consecutive blocks round-robin
-  // over the 8 XCDs; remap so each XCD walks a contiguous run of tiles and
-  // the theta/rho halos it re-reads stay in its own L2.
-  const int nblocks = gridDim.x;
-  int tile = blockIdx.x;
-  if ((nblocks & 7) == 0) tile = (blockIdx.x & 7) * (nblocks >> 3) + (blockIdx.x >> 3);
-  const int strip = tile / ngroups, grp = tile - strip * ngroups;
   const int a0 = strip * kWave;
   const int a = a0 + lane;
-  const int b0 = (grp * blockDim.y + w) * kRows;
+  const int b0 = (grp * wpb + w) * kRows;
+  if (b0 >= M) return;  // uniform over the wavefront; no block-level barrier follows
   double *L = &lds[w][0][0];
 
   const bool lo_edge = (a0 == 0), hi_edge = (a0 + kWave == N);
@@ -429,6 +453,34 @@ void k_stage(Grid g, const double *__restrict__ in, const double *base,
   }
 }
 
+// Single-solve stage launch.  XCD-aware tile order: the hardware deals
+// consecutive blocks round-robin over the 8 XCDs; remap so each XCD walks a
+// contiguous run of tiles and the theta/rho halos it re-reads stay in its
+// own L2.
+template <int INTEG, int STAGE, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W)))
+void k_stage(Grid g, DevState d, int has_src, int ngroups, int k_rel) {
+  const int nblocks = gridDim.x;
+  int tile = blockIdx.x;
+  if ((nblocks & 7) == 0) tile = (blockIdx.x & 7) * (nblocks >> 3) + (blockIdx.x >> 3);
+  const int strip = tile / ngroups, grp = tile - strip * ngroups;
+  stage_tile<INTEG, STAGE>(g, d, has_src, k_rel, strip, grp, blockDim.y);
+}
+
+// Batched stage launch: block i works on tile tiles[i] = {solve, tile} of the
+// batch (the host orders the table for the XCD dealing).  A solve that has
+// completed its scheduled steps is skipped.
+template <int INTEG, int STAGE, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W)))
+void k_stage_batch(const SolveDesc *__restrict__ descs, const int2 *__restrict__ tiles, int k_rel) {
+  const int2 t = tiles[blockIdx.x];
+  if (t.x < 0) return;  // padding of the tile table
+  const SolveDesc &sd = descs[t.x];
+  if (sd.d.ctl->k + k_rel >= sd.nt) return;
+  const int strip = t.y / sd.ngroups, grp = t.y - strip * sd.ngroups;
+  stage_tile<INTEG, STAGE>(sd.g, sd.d, sd.has_src, k_rel, strip, grp, blockDim.y);
+}
+
 // ---------------------------------------------------------------------------
 // Output records for a batch of n completed steps (body.cu:174-212): for each
 // step k with k % Fr == 0, the energy flux from the snapshot of int Psi dt
@@ -437,7 +489,7 @@ void k_stage(Grid g, const double *__restrict__ in, const double *base,
 // round-to-nearest operations in the reference's evaluation order make the
 // records equal the host computation bit for bit.  Then advances the step
 // counter by n.
-__global__ __launch_bounds__(256) void k_finalize(Grid g, Ext ext, Ctl *ctl, int n) {
+__device__ __forceinline__ void finalize_solve(const Grid &g, const Ext &ext, Ctl *ctl, int n) {
   const int M = g.M, t = threadIdx.x;
   const int k0 = ctl->k;
   const double timer0 = ctl->timer;
@@ -494,6 +546,17 @@ __global__ __launch_bounds__(256) void k_finalize(Grid g, Ext ext, Ctl *ctl, int
   }
 }
 
+__global__ __launch_bounds__(256) void k_finalize(Grid g, Ext ext, Ctl *ctl, int n) { finalize_solve(g, ext, ctl, n); }
+
+// One block per solve of the batch; a solve closes at its scheduled step count.
+__global__ __launch_bounds__(256) void k_finalize_batch(const SolveDesc *__restrict__ descs, int n) {
+  const SolveDesc &sd = descs[blockIdx.x];
+  const int left = sd.nt - sd.d.ctl->k;
+  const int n_eff = n < left ? n : left;
+  if (n_eff <= 0) return;
+  finalize_solve(sd.g, sd.d.ext, sd.d.ctl, n_eff);
+}
+
 // ---------------------------------------------------------------------------
 // Particle source on the compact ptsx x ptsy patch (kern.cu:770-2683).
 #include "teuk_source_dev.inc"
@@ -514,38 +577,94 @@ hipError_t launch_init_coeffs(const Grid &g, const DevState &d, hipStream_t s) {
   return hipGetLastError();
 }
 
-static inline int waves_per_block(int M) {
+Tiling default_tiling(int M) {
   const int chunks = M / kRows;
-  return chunks >= 4 ? 4 : (chunks >= 2 ? 2 : 1);
+  Tiling t;
+  t.wpb = chunks >= 4 ? 4 : (chunks >= 2 ? 2 : 1);
+  t.wpe = TEUK_STAGE_WAVES_PER_EU;
+  return t;
+}
+
+// k_stage<I, S, W> for the runtime (stage, wpe) pair.
+template <int I, int S>
+static hipError_t stage_one(const Grid &g, const DevState &d, int hs, int ngroups, int k_rel, dim3 grid,
+                            dim3 block, int wpe, hipStream_t s) {
+  if (wpe == 4)
+    hipLaunchKernelGGL((k_stage<I, S, 4>), grid, block, 0, s, g, d, hs, ngroups, k_rel);
+  else if (wpe == 2)
+    hipLaunchKernelGGL((k_stage<I, S, 2>), grid, block, 0, s, g, d, hs, ngroups, k_rel);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+template <int I, int S>
+static hipError_t stage_batch_one(const SolveDesc *descs, const int2 *tiles, int ntiles, int k_rel, dim3 block,
+                                  int wpe, hipStream_t s) {
+  if (wpe == 4)
+    hipLaunchKernelGGL((k_stage_batch<I, S, 4>), dim3(ntiles), block, 0, s, descs, tiles, k_rel);
+  else if (wpe == 2)
+    hipLaunchKernelGGL((k_stage_batch<I, S, 2>), dim3(ntiles), block, 0, s, descs, tiles, k_rel);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
 }
 
 hipError_t launch_stage(const Grid &g, const DevState &d, int integrator, int stage, bool has_src, int k_rel,
-                        hipStream_t s) {
-  const int wpb = waves_per_block(g.M);
-  const int ngroups = g.M / (kRows * wpb);
+                        Tiling t, hipStream_t s) {
+  if (t.wpb < 1 || t.wpb > 4 || g.M % (kRows * t.wpb) != 0) return hipErrorInvalidValue;
+  const int ngroups = g.M / (kRows * t.wpb);
   const int nb = (g.N / kWave) * ngroups;
-  const dim3 grid(nb), block(kWave, wpb);
+  const dim3 grid(nb), block(kWave, t.wpb);
   const int hs = has_src ? 1 : 0;
-#define TEUK_STAGE(I, S, IN, BASE, OUT, ACC)                                                                    \
-  hipLaunchKernelGGL((k_stage<I, S>), grid, block, 0, s, g, IN, BASE, OUT, ACC, d.ktab, d.rtab, d.dbdx, d.patch, \
-                     d.patch_org, hs, ngroups, d.ext, d.ctl, k_rel)
   if (integrator == 0) {
     switch (stage) {
-      case 1: TEUK_STAGE(0, 1, d.U, d.U, d.C, nullptr); break;
-      case 2: TEUK_STAGE(0, 2, d.C, d.U, d.D, nullptr); break;
-      case 3: TEUK_STAGE(0, 3, d.D, d.U, d.U, nullptr); break;
-      default: return hipErrorInvalidValue;
-    }
-  } else {
-    switch (stage) {
-      case 1: TEUK_STAGE(1, 1, d.U, d.U, d.C, d.A); break;
-      case 2: TEUK_STAGE(1, 2, d.C, d.U, d.D, d.A); break;
-      case 3: TEUK_STAGE(1, 3, d.D, d.U, d.C, d.A); break;
-      case 4: TEUK_STAGE(1, 4, d.C, d.U, d.U, d.A); break;
+      case 1: return stage_one<0, 1>(g, d, hs, ngroups, k_rel, grid, block, t.wpe, s);
+      case 2: return stage_one<0, 2>(g, d, hs, ngroups, k_rel, grid, block, t.wpe, s);
+      case 3: return stage_one<0, 3>(g, d, hs, ngroups, k_rel, grid, block, t.wpe, s);
       default: return hipErrorInvalidValue;
     }
   }
-#undef TEUK_STAGE
+  switch (stage) {
+    case 1: return stage_one<1, 1>(g, d, hs, ngroups, k_rel, grid, block, t.wpe, s);
+    case 2: return stage_one<1, 2>(g, d, hs, ngroups, k_rel, grid, block, t.wpe, s);
+    case 3: return stage_one<1, 3>(g, d, hs, ngroups, k_rel, grid, block, t.wpe, s);
+    case 4: return stage_one<1, 4>(g, d, hs, ngroups, k_rel, grid, block, t.wpe, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_stage_batch(const SolveDesc *descs, const int2 *tiles, int ntiles, int integrator, int stage,
+                              int k_rel, Tiling t, hipStream_t s) {
+  if (ntiles <= 0) return hipSuccess;
+  if (t.wpb < 1 || t.wpb > 4) return hipErrorInvalidValue;
+  const dim3 block(kWave, t.wpb);
+  if (integrator == 0) {
+    switch (stage) {
+      case 1: return stage_batch_one<0, 1>(descs, tiles, ntiles, k_rel, block, t.wpe, s);
+      case 2: return stage_batch_one<0, 2>(descs, tiles, ntiles, k_rel, block, t.wpe, s);
+      case 3: return stage_batch_one<0, 3>(descs, tiles, ntiles, k_rel, block, t.wpe, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
+  switch (stage) {
+    case 1: return stage_batch_one<1, 1>(descs, tiles, ntiles, k_rel, block, t.wpe, s);
+    case 2: return stage_batch_one<1, 2>(descs, tiles, ntiles, k_rel, block, t.wpe, s);
+    case 3: return stage_batch_one<1, 3>(descs, tiles, ntiles, k_rel, block, t.wpe, s);
+    case 4: return stage_batch_one<1, 4>(descs, tiles, ntiles, k_rel, block, t.wpe, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_finalize_batch(const SolveDesc *descs, int nsolves, int n, hipStream_t s) {
+  if (nsolves <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_finalize_batch, dim3(nsolves), dim3(256), 0, s, descs, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_source_batch(const SolveDesc *descs, int nsolves, int max_cells, int k_rel, hipStream_t s) {
+  if (nsolves <= 0 || max_cells <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_source_batch, dim3((max_cells + 255) / 256, nsolves), dim3(256), 0, s, descs, k_rel);
   return hipGetLastError();
 }
 
@@ -556,8 +675,7 @@ hipError_t launch_finalize(const Grid &g, const DevState &d, int n, hipStream_t 
 
 hipError_t launch_source(const Grid &g, const DevState &d, int k_rel, hipStream_t s) {
   const int cells = g.ptsx * g.ptsy;
-  hipLaunchKernelGGL(k_source, dim3((cells + 255) / 256), dim3(256), 0, s, g, d.r_c, d.rtab, d.theta, d.phase,
-                     d.sched, d.nsched, d.patch, d.patch_org, d.ctl, k_rel);
+  hipLaunchKernelGGL(k_source, dim3((cells + 255) / 256), dim3(256), 0, s, g, d, k_rel);
   return hipGetLastError();
 }
 

@@ -65,6 +65,8 @@ EXPORTS = [
     "teuk_get_fields", "teuk_set_source", "teuk_set_trajectory", "teuk_step", "teuk_sync",
     "teuk_progress", "teuk_read_output", "teuk_read_psi", "teuk_enable_timing", "teuk_timing",
     "teuk_footprint", "teuk_body", "teuk_strerror", "teuk_trajectory_schedule", "teuk_traj_plunge",
+    "teuk_set_tiling", "teuk_remaining", "teuk_batch_create", "teuk_batch_destroy", "teuk_batch_add",
+    "teuk_batch_remove", "teuk_batch_set_tiling", "teuk_batch_step", "teuk_batch_sync", "teuk_batch_info",
 ]
 
 _lib = None
@@ -108,6 +110,17 @@ def lib():
                                       ctypes.POINTER(SourceStep)], ctypes.c_int),
         "teuk_traj_plunge": ([ctypes.c_double] * 6 + [ctypes.c_ulonglong, _dp, ctypes.c_int,
                                                       ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "teuk_set_tiling": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "teuk_remaining": ([vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "teuk_batch_create": ([ctypes.c_int, ctypes.c_longlong, ctypes.POINTER(vp)], ctypes.c_int),
+        "teuk_batch_destroy": ([vp], None),
+        "teuk_batch_add": ([vp, vp], ctypes.c_int),
+        "teuk_batch_remove": ([vp, vp], ctypes.c_int),
+        "teuk_batch_set_tiling": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "teuk_batch_step": ([vp, ctypes.c_int], ctypes.c_int),
+        "teuk_batch_sync": ([vp], ctypes.c_int),
+        "teuk_batch_info": ([vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                             ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(l, name)
@@ -268,6 +281,14 @@ class Solver:
         _check(lib().teuk_read_psi(self._h, first, cap, _ptr(buf), ctypes.byref(n)), "teuk_read_psi")
         return buf[: n.value].copy()
 
+    def set_tiling(self, waves_per_block, waves_per_eu):
+        _check(lib().teuk_set_tiling(self._h, int(waves_per_block), int(waves_per_eu)), "teuk_set_tiling")
+
+    def remaining(self):
+        n = ctypes.c_int()
+        _check(lib().teuk_remaining(self._h, ctypes.byref(n)), "teuk_remaining")
+        return n.value
+
     def enable_timing(self, on=True):
         _check(lib().teuk_enable_timing(self._h, 1 if on else 0), "teuk_enable_timing")
 
@@ -275,6 +296,63 @@ class Solver:
         n = ctypes.c_longlong(); ms = ctypes.c_double()
         _check(lib().teuk_timing(self._h, ctypes.byref(n), ctypes.byref(ms)), "teuk_timing")
         return n.value, ms.value
+
+
+class Batch:
+    """Solvers stepped together by the batched kernels (a libteuk batch).
+    Holds references to its members; closing the batch leaves them valid."""
+
+    def __init__(self, integrator=1, hbm_budget=0):
+        h = ctypes.c_void_p()
+        _check(lib().teuk_batch_create(int(integrator), int(hbm_budget), ctypes.byref(h)), "teuk_batch_create")
+        self._h = h
+        self.members = []
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().teuk_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add(self, solver):
+        """Admit a solver; returns False when it would exceed the HBM budget."""
+        rc = lib().teuk_batch_add(self._h, solver._h)
+        if rc == TEUK_ENOMEM:
+            return False
+        _check(rc, "teuk_batch_add")
+        self.members.append(solver)
+        return True
+
+    def remove(self, solver):
+        _check(lib().teuk_batch_remove(self._h, solver._h), "teuk_batch_remove")
+        self.members.remove(solver)
+
+    def set_tiling(self, waves_per_block=4, waves_per_eu=2, order=0, chunk=32):
+        _check(lib().teuk_batch_set_tiling(self._h, int(waves_per_block), int(waves_per_eu), int(order),
+                                           int(chunk)), "teuk_batch_set_tiling")
+
+    def step(self, n):
+        _check(lib().teuk_batch_step(self._h, int(n)), "teuk_batch_step")
+
+    def sync(self):
+        _check(lib().teuk_batch_sync(self._h), "teuk_batch_sync")
+
+    def info(self):
+        m = ctypes.c_int(); a = ctypes.c_int(); b = ctypes.c_longlong()
+        _check(lib().teuk_batch_info(self._h, ctypes.byref(m), ctypes.byref(a), ctypes.byref(b)), "teuk_batch_info")
+        return {"members": m.value, "active": a.value, "bytes": b.value}
 
 
 def body(p, r_c, theta, traj_path=None, out_path=None, full_path=None, verbose=False):

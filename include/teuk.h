@@ -133,6 +133,42 @@ int teuk_timing(teuk_solver *s, long long *launches, double *ms);
  * schedule (admission control for batching/serving). */
 long long teuk_footprint(const teuk_params *p, int nsteps);
 
+/* Launch shape of the stage kernels: wavefronts per block (1, 2, 4; M must
+ * be a multiple of 8 x that) and the occupancy the kernels were compiled
+ * for (2 or 4 waves per SIMD).  Results do not depend on it. */
+int teuk_set_tiling(teuk_solver *s, int waves_per_block, int waves_per_eu);
+
+/* Steps left in the solver's schedule ((int)(T/dt), or the source schedule
+ * if shorter). */
+int teuk_remaining(teuk_solver *s, int *steps);
+
+/* ---- batches ------------------------------------------------------------------ */
+/* No counterpart in the reference (body() solves one mode per call; the
+ * reference runs modes and trajectories as separate jobs, main.c:50-55,
+ * run.sh:3-16).  A batch steps many solvers together: each RK stage of every
+ * member is one launch (blocks index a table of {solver, tile}); chunks of
+ * steps are captured in a hipGraph.  Members may differ in N, M, a, m, dt,
+ * source and length; each stops at its own schedule end.  A member's results
+ * are bitwise those of stepping it alone.  Members must share the
+ * integrator.  While in a batch a solver is stepped only by its batch;
+ * teuk_read_output / teuk_read_psi / teuk_get_fields / teuk_progress still
+ * work on it. */
+typedef struct teuk_batch teuk_batch;
+/* hbm_budget: bytes of member footprint (teuk_footprint) the batch admits;
+ * 0 = no limit. */
+int teuk_batch_create(int integrator, long long hbm_budget, teuk_batch **out);
+void teuk_batch_destroy(teuk_batch *b);      /* members stay valid */
+/* TEUK_ENOMEM when the member would exceed the budget. */
+int teuk_batch_add(teuk_batch *b, teuk_solver *s);
+int teuk_batch_remove(teuk_batch *b, teuk_solver *s);
+/* order 0: each XCD gets a contiguous run of tiles; 1: tiles dealt in list
+ * order.  chunk: steps per captured graph. */
+int teuk_batch_set_tiling(teuk_batch *b, int waves_per_block, int waves_per_eu, int order, int chunk);
+/* Advance every unfinished member by up to nsteps steps (asynchronous). */
+int teuk_batch_step(teuk_batch *b, int nsteps);
+int teuk_batch_sync(teuk_batch *b);
+int teuk_batch_info(teuk_batch *b, int *members, int *active, long long *bytes_used);
+
 /* ---- drop-in for body() ---------------------------------------------------- */
 /* body.cu:25-225: one full m-mode solve.  Reads the trajectory file
  * (traj_path, may be NULL for no source), evolves from zero fields to T,

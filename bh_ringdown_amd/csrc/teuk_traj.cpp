@@ -6,10 +6,10 @@
 // Model (a compact Ori-Thorne-style transition, not a port of GenOT): the
 // particle starts on the circular orbit of fixed inclination iota at
 // r0 = f r_isco(iota), loses energy and angular momentum at the quadrupole
-// rates of a circular orbit with the same frequency, dE/dt = -(32/5) eta
-// Omega^(10/3), dLz/dt = cos(iota) dE/dt / Omega, with Q = Lz^2 tan^2(iota)
+// rates of a circular orbit with the same angular speed Omega,
+// dE/dt = -(32/5) eta Omega^(10/3), dLz/dt = cos(iota) dE/dt / Omega, with Q = Lz^2 tan^2(iota)
 // (fixed inclination); once r falls below r_isco - 0.5 M the constants are
-// frozen and the motion is the geodesic plunge.  Positions follow Kerr's
+// frozen and the motion relaxes onto the geodesic plunge.  Positions follow Kerr's
 // second-order geodesic equations in Mino time, integrated by RK4 in BL time:
 //   (dr/dl)^2 = R(r) = [E(r^2+a^2) - a Lz]^2 - Delta [r^2 + (Lz - a E)^2 + Q],
 //   d^2r/dl^2 = R'(r)/2, so r tracks the moving potential minimum while E and
@@ -28,9 +28,18 @@
 
 namespace {
 
+// E, Lz and Carter's Q; Lt is the total angular momentum of the orbit's
+// inclination convention, Lz = Lt cos(iota), Q = Lt^2 sin^2(iota) (so
+// Q = Lz^2 tan^2(iota), defined up to polar orbits).
 struct Consts {
-  double E, L, Q;
+  double E, L, Q, Lt;
 };
+
+void set_lt(Consts &c, double Lt, double ci, double si) {
+  c.Lt = Lt;
+  c.L = Lt * ci;
+  c.Q = Lt * Lt * si * si;
+}
 
 double Rfun(double r, double a, const Consts &c) {
   const double P = c.E * (r * r + a * a) - a * c.L;
@@ -47,33 +56,28 @@ double d2Rdr2(double r, double a, const Consts &c) {
   return (dRdr(r + h, a, c) - dRdr(r - h, a, c)) / (2.0 * h);
 }
 
-// Circular orbit of inclination iota at radius r: R = R' = 0 with
-// Q = L^2 tan^2 iota, by Newton from the equatorial prograde constants.
+// Circular orbit of inclination iota at radius r: R = R' = 0, by Newton in
+// (E, Lt) from the equatorial prograde constants.
 bool circular(double r, double a, double iota, Consts &c) {
   const double v = 1.0 / std::sqrt(r);
   const double den = std::sqrt(1.0 - 3.0 * v * v + 2.0 * a * v * v * v);
+  const double ci = std::cos(iota), si = std::sin(iota);
   c.E = (1.0 - 2.0 * v * v + a * v * v * v) / den;
-  c.L = std::sqrt(r) * (1.0 - 2.0 * a * v * v * v + a * a * v * v * v * v) / den * std::cos(iota);
-  const double t2 = std::tan(iota) * std::tan(iota);
+  set_lt(c, std::sqrt(r) * (1.0 - 2.0 * a * v * v * v + a * a * v * v * v * v) / den, ci, si);
   for (int it = 0; it < 100; it++) {
-    c.Q = c.L * c.L * t2;
     const double f1 = Rfun(r, a, c), f2 = dRdr(r, a, c);
     const double hE = 1e-7, hL = 1e-7;
     Consts cE = c, cL = c;
     cE.E += hE;
-    cL.L += hL;
-    cL.Q = cL.L * cL.L * t2;
+    set_lt(cL, c.Lt + hL, ci, si);
     const double j11 = (Rfun(r, a, cE) - f1) / hE, j12 = (Rfun(r, a, cL) - f1) / hL;
     const double j21 = (dRdr(r, a, cE) - f2) / hE, j22 = (dRdr(r, a, cL) - f2) / hL;
     const double det = j11 * j22 - j12 * j21;
     if (det == 0.0) return false;
     const double dE = (f1 * j22 - f2 * j12) / det, dL = (j11 * f2 - j21 * f1) / det;
     c.E -= dE;
-    c.L -= dL;
-    if (std::fabs(dE) < 1e-14 && std::fabs(dL) < 1e-13) {
-      c.Q = c.L * c.L * t2;
-      return true;
-    }
+    set_lt(c, c.Lt - dL, ci, si);
+    if (std::fabs(dE) < 1e-14 && std::fabs(dL) < 1e-13) return true;
   }
   return false;
 }
@@ -117,12 +121,18 @@ Deriv rhs(const State &s, double a, const Consts &c) {
   return d;
 }
 
+// Polar passes (theta -> 0) are excluded: the (theta, phi) description of
+// the worldline and of the source is singular on the axis.
+constexpr double kMaxIota = 85.0 * M_PI / 180.0;
+// Time constant (M) of the velocity relaxation after the constants freeze.
+constexpr double kRelax = 2.0;
+
 }  // namespace
 
 extern "C" int teuk_traj_plunge(double a, double eta, double iota, double r0_over_isco, double dt_out,
                                 double t_final, unsigned long long seed, double *rows, int cap, int *nrows) {
   if (!rows || !nrows || cap <= 0 || !(a >= 0.0 && a < 1.0) || !(eta > 0.0) || !(dt_out > 0.0) ||
-      !(t_final > 0.0) || !(r0_over_isco >= 1.0))
+      !(t_final > 0.0) || !(r0_over_isco >= 1.0) || !(iota >= 0.0 && iota <= kMaxIota))
     return TEUK_EINVAL;
   const double rh = 1.0 + std::sqrt(1.0 - a * a);
   const double risco = isco(a, iota);
@@ -171,7 +181,6 @@ extern "C" int teuk_traj_plunge(double a, double eta, double iota, double r0_ove
     n++;
   };
   emit(s);
-  const double t2 = std::tan(iota) * std::tan(iota);
   while (n < cap && n * dt_out < t_final + 0.5 * dt_out) {
     for (int k = 0; k < sub; k++) {
       // RK4 in BL time: dy/dt = (dy/dl) / (dt/dl)
@@ -201,18 +210,21 @@ extern "C" int teuk_traj_plunge(double a, double eta, double iota, double r0_ove
       s = ns;
       // radiation reaction until the plunge
       if (!plunging) {
+        // angular speed on the sphere, sqrt(phidot^2 sin^2 + thetadot^2)
         const Deriv d = rhs(s, a, c);
-        const double Om = std::fabs(d.dphi / d.dt);
+        const double sn2 = std::max(1e-300, 1.0 - s.z * s.z);
+        const double phd = d.dphi / d.dt, zd = d.dz / d.dt;
+        const double Om = std::sqrt(phd * phd * sn2 + zd * zd / sn2);
         const double dE = -(32.0 / 5.0) * eta * std::pow(Om, 10.0 / 3.0) * h;
         c.E += dE;
-        c.L += std::cos(iota) * dE / Om;
-        c.Q = c.L * c.L * t2;
+        set_lt(c, c.Lt + dE / Om, std::cos(iota), std::sin(iota));
         if (s.r < risco - 0.5) plunging = true;
-        // keep the radial first integral (dr/dl)^2 = R(r) under the changing
-        // constants, so the frozen plunge starts on its geodesic
+      } else if (s.ur < 0.0) {
+        // With E and Lz changing, (dr/dl)^2 - R(r) drifts from zero; after the
+        // constants freeze, relax dr/dl onto the plunge geodesic's
+        // -sqrt(R(r)) over a few M of time so the velocity stays continuous.
         const double Rv = Rfun(s.r, a, c);
-        const double v2 = s.ur * s.ur;
-        if (Rv > 0.0 && v2 > 0.0) s.ur *= std::sqrt(Rv / v2);
+        if (Rv > 0.0) s.ur += (-std::sqrt(Rv) - s.ur) * std::min(1.0, h / kRelax);
       }
     }
     s.t = n * dt_out;  // rows sit on the exact output grid

@@ -1,0 +1,170 @@
+"""Autotuner for the stage-kernel launch shape and the batch schedule.
+
+Candidates (all give bitwise-identical results; only speed differs):
+  waves_per_block  1, 2, 4  -- wavefronts (64 rho columns x 8 theta rows each)
+                               stacked along theta in one block
+  waves_per_eu     2, 4     -- occupancy the kernel is compiled for (4 forces
+                               register spilling; kept as a candidate because
+                               the trade-off is measured, not assumed)
+  order            0, 1     -- batch tile table: contiguous run per XCD, or
+                               dealt in list order
+  chunk            16, 32, 64 -- steps per captured hipGraph
+
+tune_solver() times single-solve stage launches with HIP events;
+tune_batch() times whole batch steps.  Winners are kept in a JSON cache
+(TEUK_TUNE_CACHE, default ~/.cache/bh_ringdown_amd/tuning.json) keyed by
+grid shape and integrator, and looked up by best_solver_tiling() /
+best_batch_tiling().
+"""
+import itertools
+import json
+import os
+import time
+
+import numpy as np
+
+from . import teuk
+
+CACHE = os.environ.get("TEUK_TUNE_CACHE") or os.path.join(os.path.expanduser("~"), ".cache", "bh_ringdown_amd",
+                                                          "tuning.json")
+
+
+def _load():
+    try:
+        with open(CACHE) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(key, value):
+    data = _load()
+    data[key] = value
+    os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+    tmp = CACHE + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+    os.replace(tmp, CACHE)
+
+
+def solver_key(N, M, integrator):
+    return f"solver/{N}x{M}/i{integrator}"
+
+
+def batch_key(shapes, integrator):
+    return "batch/" + ",".join(f"{n}x{m}" for n, m in sorted(set(shapes))) + f"/i{integrator}"
+
+
+def _probe_params(N, M, integrator, steps):
+    # a stable, source-free probe grid of the right shape (dt / dtheta <= 2)
+    p = teuk.default_params(N, M, -0.04 * N, 0.06 * N, 0.7, 2, dt_over_dx=0.5)
+    p.T = (steps + 1) * p.dt
+    p.integrator = integrator
+    return p
+
+
+def tune_solver(N, M, integrator=1, steps=60, repeats=2, store=True):
+    """Time every (waves_per_block, waves_per_eu) on an N x M grid; returns
+    (best, {candidate: us per stage launch})."""
+    times = {}
+    for wpb, wpe in itertools.product((1, 2, 4), (2, 4)):
+        if M % (8 * wpb):
+            continue
+        p = _probe_params(N, M, integrator, steps * repeats + 10)
+        with teuk.Solver(p) as s:
+            s.set_tiling(wpb, wpe)
+            s.step(5)
+            s.sync()
+            s.enable_timing(True)
+            for _ in range(repeats):
+                s.step(steps)
+            s.sync()
+            n, ms = s.timing()
+        times[(wpb, wpe)] = 1e3 * ms / n
+    best = min(times, key=times.get)
+    if store:
+        _store(solver_key(N, M, integrator), {"waves_per_block": best[0], "waves_per_eu": best[1],
+                                              "us_per_launch": times[best]})
+    return best, times
+
+
+def tune_batch(shapes, integrator=1, steps=64, store=True, chunks=(16, 32, 64)):
+    """Time whole batch steps of probe solvers of the given (N, M) shapes for
+    every (waves_per_block, waves_per_eu, order, chunk); returns
+    (best, {candidate: seconds per step})."""
+    times = {}
+    for wpb, wpe, order, chunk in itertools.product((1, 2, 4), (2, 4), (0, 1), chunks):
+        if any(m % (8 * wpb) for _, m in shapes):
+            continue
+        solvers = [teuk.Solver(_probe_params(n, m, integrator, 3 * steps + 10)) for n, m in shapes]
+        try:
+            with teuk.Batch(integrator=integrator) as b:
+                b.set_tiling(wpb, wpe, order, chunk)
+                for s in solvers:
+                    b.add(s)
+                b.step(chunk)
+                b.sync()
+                t0 = time.perf_counter()
+                b.step(2 * steps)
+                b.sync()
+                times[(wpb, wpe, order, chunk)] = (time.perf_counter() - t0) / (2 * steps)
+        finally:
+            for s in solvers:
+                s.close()
+    best = min(times, key=times.get)
+    if store:
+        _store(batch_key(shapes, integrator), {"waves_per_block": best[0], "waves_per_eu": best[1],
+                                               "order": best[2], "chunk": best[3], "s_per_step": times[best]})
+    return best, times
+
+
+def best_solver_tiling(N, M, integrator=1):
+    """Tuned (waves_per_block, waves_per_eu) for a grid, or None."""
+    e = _load().get(solver_key(N, M, integrator))
+    return (e["waves_per_block"], e["waves_per_eu"]) if e else None
+
+
+def best_batch_tiling(shapes, integrator=1):
+    """Tuned batch schedule as set_tiling keyword arguments, or None.  Falls
+    back to the entry for the same set of row counts M if the exact shape
+    set was not tuned."""
+    data = _load()
+    e = data.get(batch_key(shapes, integrator))
+    if e is None:
+        ms = sorted({m for _, m in shapes})
+        for k, v in data.items():
+            if k.startswith("batch/") and k.endswith(f"/i{integrator}"):
+                kms = sorted({int(x.split("x")[1]) for x in k.split("/")[1].split(",")})
+                if kms == ms:
+                    e = v
+                    break
+    if not e:
+        return None
+    return {"waves_per_block": e["waves_per_block"], "waves_per_eu": e["waves_per_eu"], "order": e["order"],
+            "chunk": e["chunk"]}
+
+
+def main(argv=None):
+    """python -m bh_ringdown_amd.tune [--solver NxM ...] [--batch NxM,NxM,...]"""
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--solver", nargs="*", default=[])
+    ap.add_argument("--batch", nargs="*", default=[])
+    ap.add_argument("--integrator", type=int, default=1)
+    args = ap.parse_args(argv)
+    for s in args.solver:
+        N, M = map(int, s.split("x"))
+        best, t = tune_solver(N, M, args.integrator)
+        print(json.dumps({"solver": s, "best": best, "us": {f"{k[0]},{k[1]}": round(v, 3) for k, v in t.items()}}),
+              flush=True)
+    for spec in args.batch:
+        shapes = [tuple(map(int, x.split("x"))) for x in spec.split(",")]
+        best, t = tune_batch(shapes, args.integrator)
+        print(json.dumps({"batch": spec, "best": best,
+                          "ms_per_step": {",".join(map(str, k)): round(1e3 * v, 4) for k, v in t.items()}}),
+              flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

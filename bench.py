@@ -22,6 +22,8 @@ import re
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -50,7 +52,6 @@ def cpu_baseline(p, rows, seconds=12.0):
     workload: source, the RK stages, boundaries, extraction and projection
     per step.  The one-off coefficient set-up is timed separately and
     subtracted (a one-step run)."""
-    import numpy as np
     from bh_ringdown_amd import teuk
     from oracle import oracle as O
     nth = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
@@ -79,6 +80,74 @@ def cpu_baseline(p, rows, seconds=12.0):
                       f"{p.N}x{p.M} plunge solve; oracle g++ -O3 OpenMP, set-up excluded"}
 
 
+def batch_section(scale, log_every=True):
+    """The 72 uneven solves (9 m x 8 spins, grids by spin) at `scale` of their
+    final time: the plain one-solve-at-a-time loop against one batch."""
+    from bh_ringdown_amd import teuk, tune, workloads
+    items = workloads.batch_72(seed=0, t_final=1200.0 * scale)
+    total = sum(p.N * p.M * p.nsteps() for p, _ in items)
+    log(f"bench: batch of {len(items)} solves, {total:.3e} point-updates")
+    probe = {0, 31, 71}
+    t0 = time.perf_counter()
+    plain_out = {}
+    for i, (p, rows) in enumerate(items):
+        with teuk.Solver(p) as s:
+            s.set_trajectory(rows, p.nsteps())
+            s.step(p.nsteps())
+            s.sync()
+            if i in probe:
+                plain_out[i] = s.output()["h"]
+    t_plain = time.perf_counter() - t0
+    log(f"bench: plain loop {t_plain:.2f} s")
+    shapes = [(p.N, p.M) for p, _ in items]
+    tiling = tune.best_batch_tiling(shapes) or {"waves_per_block": 4, "waves_per_eu": 2, "order": 0, "chunk": 32}
+    t0 = time.perf_counter()
+    solvers = []
+    with teuk.Batch(integrator=1) as b:
+        b.set_tiling(**tiling)
+        for p, rows in items:
+            s = teuk.Solver(p)
+            s.set_trajectory(rows, p.nsteps())
+            b.add(s)
+            solvers.append(s)
+        b.step(max(p.nsteps() for p, _ in items))
+        b.sync()
+        same = all(np.array_equal(solvers[i].output()["h"], plain_out[i]) for i in probe)
+    for s in solvers:
+        s.close()
+    t_batch = time.perf_counter() - t0
+    log(f"bench: batched {t_batch:.2f} s")
+    return {"workload": f"72 solves (m -4..4 x 8 seeded spins; N x M by spin in 4096x32, 8192x32, "
+                        f"16384x64) to t_final {1200.0 * scale:g} ({scale:g} of 1200), RK4, plunge source",
+            "solves_per_hour": 3600.0 * len(items) / t_batch, "plain_solves_per_hour": 3600.0 * len(items) / t_plain,
+            "speedup_vs_plain": t_plain / t_batch, "point_updates_per_s": total / t_batch,
+            "plain_point_updates_per_s": total / t_plain, "tiling": tiling, "members_bitwise_equal_alone": same}
+
+
+def serve_section(scale, seconds):
+    """A seeded Poisson stream (20 requests/s for `seconds`, request lengths
+    scaled by `scale`) through the continuous-batching server, and the same
+    requests through the plain loop."""
+    from bh_ringdown_amd import serve
+    reqs = serve.poisson_requests(20.0, seconds, seed=0, t_range=(500.0 * scale, 2000.0 * scale))
+    log(f"bench: serving {len(reqs)} requests")
+    srv = serve.Server(hbm_budget=64 << 30, chunk=32)
+    t0 = time.perf_counter()
+    res_s = srv.run(reqs)
+    wall_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    res_p = serve.run_plain(reqs)
+    wall_p = time.perf_counter() - t0
+    same = all(np.array_equal(res_s[r.id].records["h"], res_p[r.id].records["h"]) for r in reqs)
+    ss, sp = serve.summary(res_s, wall_s), serve.summary(res_p, wall_p)
+    log(f"bench: served {wall_s:.2f} s, plain {wall_p:.2f} s")
+    return {"workload": f"Poisson 20 req/s for {seconds:g} s (seed 0, {len(reqs)} requests), a~U[0,0.99), "
+                        f"m in 1..4, iota~U[0,pi/2) capped 85 deg, q log-uniform [1e-4,1e-2], N in 4096/8192/16384, "
+                        f"M in 32/64, t_final~U[{500 * scale:g},{2000 * scale:g}) ({scale:g} of the full lengths), RK4",
+            "served": ss, "plain": sp, "solves_per_hour_speedup": ss["solves_per_hour"] / sp["solves_per_hour"],
+            "p99_ratio_plain_over_served": sp["p99_s"] / ss["p99_s"], "results_bitwise_equal_alone": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +158,10 @@ def main():
     ap.add_argument("--integrator", choices=["rk4", "rk3"], default="rk4")
     ap.add_argument("--wpb", type=int, default=0, help="wavefronts per block (0 = default)")
     ap.add_argument("--wpe", type=int, default=0, help="occupancy target, waves per SIMD (0 = default)")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--scale", type=float, default=0.1, help="fraction of the batch/serve solve lengths")
+    ap.add_argument("--serve-seconds", type=float, default=1.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -151,6 +224,10 @@ def main():
                      "kernel": "k_stage (fused RK stage)", "bytes_per_point_update": b_alg,
                      "avg_launch_us": per_launch_s * 1e6},
     }
+    if not args.no_batch:
+        out["batch"] = batch_section(args.scale)
+    if not args.no_serve:
+        out["serving"] = serve_section(args.scale, args.serve_seconds)
     if not args.no_cpu:
         log("bench: cpu baseline")
         out["cpu_baseline"] = cpu_baseline(p, rows)

@@ -60,3 +60,35 @@ def plunge_rows(p, nsteps, eta=1e-3, iota_deg=30.0, seed=0, r0_over_isco=1.02):
     t_final = (2 * nsteps + 4) * (p.dt / 2)
     return teuk.plunge_trajectory(p.a, eta, math.radians(iota_deg), p.dt / 2, t_final, seed=seed,
                                   r0_over_isco=r0_over_isco)
+
+
+def grid_for_spin(a):
+    """Grid size by spin for the batched set: higher spins need more rho
+    resolution near the horizon (and theta resolution at the top)."""
+    if a < 0.4:
+        return 4096, 32
+    if a < 0.8:
+        return 8192, 32
+    return 16384, 64
+
+
+def batch_72(seed=0, t_final=1200.0, integrator=1, eta=1e-3, iota_deg=30.0):
+    """Batched uneven solves: m in -4..4 crossed with 8 spins a ~ U[0, 0.99)
+    drawn from the seed, each spin's grid from grid_for_spin() on rho in
+    [-400, 1200], dt = 0.5 drho, to t_final, all with the same inclined
+    plunge (mass ratio eta, inclination iota) in each spin's geometry.
+    Returns [(params, trajectory rows)] (72 entries)."""
+    rng = np.random.default_rng(seed)
+    spins = rng.uniform(0.0, 0.99, size=8)
+    out = []
+    for a in spins:
+        N, M = grid_for_spin(a)
+        rows = None
+        for m in range(-4, 5):
+            p = teuk.default_params(N, M, -400.0, 1200.0, float(a), m, dt_over_dx=0.5, T=t_final)
+            p.integrator = integrator
+            p.Fr = 2
+            if rows is None:
+                rows = plunge_rows(p, p.nsteps(), eta=eta, iota_deg=iota_deg, seed=seed)
+            out.append((p, rows))
+    return out

@@ -32,6 +32,7 @@
 #include "teuk_internal.h"
 
 #include <climits>
+#include <type_traits>
 
 namespace teuk {
 namespace {
@@ -149,7 +150,9 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
   __shared__ double lds[4][4][kLdsCols];
   const int N = g.N, M = g.M;
   const size_t NM = (size_t)N * M;
-  const int lane = threadIdx.x, w = threadIdx.y;
+  // The wavefront index is uniform: make that visible to the compiler so the
+  // row index b is a scalar and the per-row tables are scalar loads.
+  const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const StageIO io = stage_io<INTEG, STAGE>(d);
   const double *__restrict__ in = io.in;
   const double *base = io.base;
@@ -239,65 +242,79 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
       }
     }
   };
-  // Store one finished row; in the last stage the extraction-column lane
-  // also advances that row's time integrals (body.cu:177-187).
+  // Store one finished row (no conditions: the compiler's memory wait counts
+  // stay exact across rows).
   auto store_row = [&](int b, const double (&v)[4]) {
     const size_t c = (size_t)b * N + a;
 #pragma unroll
     for (int f = 0; f < 4; f++) out[f * NM + c] = v[f];
-    if (kLast && a == g.je) {
-      const double ir = ext.integ[b] + v[0] * dt;
-      const double ii = ext.integ[M + b] + v[1] * dt;
-      const double dr = ext.integ[2 * M + b] + ir * dt;
-      const double di = ext.integ[3 * M + b] + ii * dt;
-      ext.integ[b] = ir;
-      ext.integ[M + b] = ii;
-      ext.integ[2 * M + b] = dr;
-      ext.integ[3 * M + b] = di;
-      if (!isfinite(v[0]) || !isfinite(v[1])) ctl->status = 1;
-      if (snap) {
-        double *sn = ext.snap + (size_t)(kstep / g.Fr) * 6 * M;
-        sn[b] = v[0];
-        sn[M + b] = v[1];
-        sn[2 * M + b] = ir;
-        sn[3 * M + b] = ii;
-        sn[4 * M + b] = dr;
-        sn[5 * M + b] = di;
-      }
-    }
   };
 
-  // Per-row global inputs, fetched one row ahead: the LDS fences below keep
-  // the compiler from hoisting loads across rows, so the next row's loads
-  // are issued before the current row's fences and overlap its arithmetic.
+  // Global inputs of a row.  The stencil inputs (p and the halo columns) are
+  // fetched one row ahead; the inputs used only after the LDS stencils
+  // (stage base, RK4 accumulator, dbdx, source patch, row values) are issued at the top
+  // of their own row, before the next row's prefetch, so that waiting for
+  // them never waits for the prefetch (loads retire in issue order).
   struct RowIn {
-    double pr, pi, halo[4], base[4], dbdx;
+    double pr, pi, halo[4];
+  };
+  struct RowLate {
+    double base[4], acc[4], dbdx, sr, si, rt[kNR];  // rt: the row's sin^2, cos, Y, cot
   };
   const int slot = lane < kHalo ? lane : kWave + lane;  // halo slot of lanes 0..7
   const int hcol = a0 - kHalo + slot;
   const bool halo_lane = lane < 2 * kHalo && hcol >= 0 && hcol < N;
+  // Every load below is issued by all lanes from an in-bounds address (lanes
+  // without a halo column or outside the patch re-read their own column, a
+  // cache hit, and discard the value): no branches around loads, so the
+  // compiler's wait counts stay exact and never cover the prefetch.
+  const int hsafe = halo_lane ? hcol : a;
+  // ... and every lane writes its halo value to LDS: a lane without a halo
+  // column writes its own column's value (the same bits) to its own slot.
+  const int hslot = halo_lane ? slot : lane + kHalo;
   constexpr bool kBase = (INTEG == 0 && STAGE >= 2) || (INTEG == 1 && (STAGE == 2 || STAGE == 3));
+  constexpr bool kAccIn = INTEG == 1 && STAGE >= 2;
   auto load_row = [&](int b) {
     RowIn x;
     const size_t c0 = (size_t)b * N;
     x.pr = in[2 * NM + c0 + a];
     x.pi = in[3 * NM + c0 + a];
 #pragma unroll
-    for (int f = 0; f < 4; f++) x.halo[f] = halo_lane ? in[f * NM + c0 + hcol] : 0.0;
-#pragma unroll
-    for (int f = 0; f < 4; f++) x.base[f] = kBase ? base[f * NM + c0 + a] : 0.0;
-    x.dbdx = dbdx[c0 + a];
+    for (int f = 0; f < 4; f++) x.halo[f] = in[f * NM + c0 + hsafe];
     return x;
   };
-  RowIn cur = load_row(b0 < 3 ? 3 : b0);
-
+  auto load_late = [&](int b) {
+    RowLate x;
+    const size_t c = (size_t)b * N + a;
 #pragma unroll
-  for (int i = 0; i < kRows; i++) {
+    for (int f = 0; f < 4; f++) x.base[f] = kBase ? base[f * NM + c] : 0.0;
+#pragma unroll
+    for (int f = 0; f < 4; f++) x.acc[f] = kAccIn ? acc[f * NM + c] : 0.0;
+    x.dbdx = dbdx[c];
+#pragma unroll
+    for (int j = 0; j < kNR; j++) x.rt[j] = rtab[j * M + b];
+    const int sb = b - it0, sa = a - ip0;
+    const bool in_patch = (unsigned)sb < (unsigned)g.ptsy && (unsigned)sa < (unsigned)g.ptsx;
+    const int pc = in_patch ? sb * g.ptsx + sa : 0;
+    const double pr_ = patch[pc], pi_ = patch[g.ptsx * g.ptsy + pc];
+    x.sr = in_patch ? pr_ : 0.0;
+    x.si = in_patch ? pi_ : 0.0;
+    return x;
+  };
+  // Rows of the chunk.  Chunks that hold a theta-edge row skip the rows the
+  // boundary fill produces; all other chunks run the same body with no
+  // row conditions at all (exact wait counts, prefetch always in flight).
+  auto run_rows = [&](auto edge_tag) {
+    constexpr bool kEdge = decltype(edge_tag)::value;
+    RowIn cur = load_row(kEdge && b0 < 3 ? 3 : b0);
+#pragma unroll
+    for (int i = 0; i < kRows; i++) {
     const int b = b0 + i;
-    if (b < 3 || b >= M - 3) continue;  // uniform over the wavefront
+    if (kEdge && (b < 3 || b >= M - 3)) continue;  // uniform over the wavefront
     const size_t c0 = (size_t)b * N;
+    const RowLate late = load_late(b);
     RowIn nxt = cur;
-    if (i + 1 < kRows && b + 1 < M - 3) nxt = load_row(b + 1);
+    if (i + 1 < kRows) nxt = load_row(kEdge && b + 1 >= M - 3 ? b : b + 1);
     const double pr0 = cur.pr, pi0 = cur.pi;
     const double qr0 = wqr[i + 3], qi0 = wqi[i + 3];
 
@@ -306,10 +323,8 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
     L[1 * kLdsCols + lane + kHalo] = qi0;
     L[2 * kLdsCols + lane + kHalo] = pr0;
     L[3 * kLdsCols + lane + kHalo] = pi0;
-    if (halo_lane) {
 #pragma unroll
-      for (int f = 0; f < 4; f++) L[f * kLdsCols + slot] = cur.halo[f];
-    }
+    for (int f = 0; f < 4; f++) L[f * kLdsCols + hslot] = cur.halo[f];
     wave_sync();
 
     double v[4] = {0, 0, 0, 0};
@@ -324,7 +339,7 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
         return r[c + 4] + r[c - 4] - 8.0 * (r[c + 3] + r[c - 3]) + 70.0 * r[c] - 56.0 * (r[c + 1] + r[c - 1]) +
                28.0 * (r[c + 2] + r[c - 2]);
       };
-      const double ct = rtab[3 * M + b];
+      const double ct = late.rt[3];
       const double qry = ((wqr[i + 6] - wqr[i]) - 9.0 * (wqr[i + 5] - wqr[i + 1]) + 45.0 * (wqr[i + 4] - wqr[i + 2])) * wy;
       const double qryy = (2.0 * (wqr[i + 6] + wqr[i]) - 27.0 * (wqr[i + 5] + wqr[i + 1]) +
                            270.0 * (wqr[i + 4] + wqr[i + 2]) - 490.0 * wqr[i + 3]) * wyy;
@@ -347,15 +362,9 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
       dpi = 0.1 * dpi * 0.0625 * over_dt;
 
       const size_t id = c0 + a;
-      const Coef cf = coef_point(K, rtab[b], rtab[M + b], rtab[2 * M + b]);
-      const double dre = cf.dre - cur.dbdx * cf.ee;
-
-      double sr = 0.0, si = 0.0;
-      const int sb = b - it0, sa = a - ip0;
-      if ((unsigned)sb < (unsigned)g.ptsy && (unsigned)sa < (unsigned)g.ptsx) {
-        sr = patch[sb * g.ptsx + sa];
-        si = patch[g.ptsx * g.ptsy + sb * g.ptsx + sa];
-      }
+      const Coef cf = coef_point(K, late.rt[0], late.rt[1], late.rt[2]);
+      const double dre = cf.dre - late.dbdx * cf.ee;
+      const double sr = late.sr, si = late.si;
 
       double r[4];
       r[0] = -cf.bb * qrx + pr0 - dqr;
@@ -372,10 +381,10 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
           for (int f = 0; f < 4; f++) v[f] = u[f] + dt * r[f];
         } else if (STAGE == 2) {
 #pragma unroll
-          for (int f = 0; f < 4; f++) v[f] = 0.25 * (3.0 * cur.base[f] + u[f] + dt * r[f]);
+          for (int f = 0; f < 4; f++) v[f] = 0.25 * (3.0 * late.base[f] + u[f] + dt * r[f]);
         } else {
 #pragma unroll
-          for (int f = 0; f < 4; f++) v[f] = kThird * (cur.base[f] + 2.0 * u[f] + 2.0 * dt * r[f]);
+          for (int f = 0; f < 4; f++) v[f] = kThird * (late.base[f] + 2.0 * u[f] + 2.0 * dt * r[f]);
         }
       } else {
         const double h2 = 0.5 * dt, h6 = dt / 6.0, h3 = dt / 3.0;
@@ -389,12 +398,12 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
           const double h = (STAGE == 2) ? h2 : dt;
 #pragma unroll
           for (int f = 0; f < 4; f++) {
-            v[f] = cur.base[f] + h * r[f];
-            acc[f * NM + id] = acc[f * NM + id] + h3 * r[f];
+            v[f] = late.base[f] + h * r[f];
+            acc[f * NM + id] = late.acc[f] + h3 * r[f];
           }
         } else {
 #pragma unroll
-          for (int f = 0; f < 4; f++) v[f] = acc[f * NM + id] + h6 * r[f];
+          for (int f = 0; f < 4; f++) v[f] = late.acc[f] + h6 * r[f];
         }
       }
     }
@@ -402,15 +411,22 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
     rho_fix(v);
     store_row(b, v);
     cur = nxt;
-    if (b == 3 || b == M - 5) {
+    if (kEdge) {
+      if (b == 3 || b == M - 5) {
 #pragma unroll
-      for (int f = 0; f < 4; f++) lo[f] = v[f];
-    }
-    if (b == 4 || b == M - 4) {
+        for (int f = 0; f < 4; f++) lo[f] = v[f];
+      }
+      if (b == 4 || b == M - 4) {
 #pragma unroll
-      for (int f = 0; f < 4; f++) hi[f] = v[f];
+        for (int f = 0; f < 4; f++) hi[f] = v[f];
+      }
     }
-  }
+    }
+  };
+  if (b0 >= 3 && b0 + kRows <= M - 3)
+    run_rows(std::false_type{});
+  else
+    run_rows(std::true_type{});
 
   // Theta edge rows (kern.cu:320-409), then their rho edges.
   if (b0 == 0) {
@@ -450,6 +466,34 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
     store_row(M - 3, s3);
     store_row(M - 2, s2);
     store_row(M - 1, s1);
+  }
+
+  // Last stage: the extraction-column lane advances the time integrals of
+  // its chunk's rows (body.cu:177-187) from the values it just stored, and
+  // snapshots them every Fr steps.
+  if (kLast && g.je >= a0 && g.je < a0 + kWave && a == g.je) {
+    for (int b = b0; b < b0 + kRows; b++) {
+      const size_t c = (size_t)b * N + a;
+      const double q0 = out[c], q1 = out[NM + c];
+      const double ir = ext.integ[b] + q0 * dt;
+      const double ii = ext.integ[M + b] + q1 * dt;
+      const double dr = ext.integ[2 * M + b] + ir * dt;
+      const double di = ext.integ[3 * M + b] + ii * dt;
+      ext.integ[b] = ir;
+      ext.integ[M + b] = ii;
+      ext.integ[2 * M + b] = dr;
+      ext.integ[3 * M + b] = di;
+      if (!isfinite(q0) || !isfinite(q1)) ctl->status = 1;
+      if (snap) {
+        double *sn = ext.snap + (size_t)(kstep / g.Fr) * 6 * M;
+        sn[b] = q0;
+        sn[M + b] = q1;
+        sn[2 * M + b] = ir;
+        sn[3 * M + b] = ii;
+        sn[4 * M + b] = dr;
+        sn[5 * M + b] = di;
+      }
+    }
   }
 }
 

@@ -930,3 +930,13 @@ int teuk_remaining(teuk_solver *s, int *steps) {
   *steps = member_steps(s) - s->k_host;
   return TEUK_OK;
 }
+
+int teuk_source_coeffs(teuk_solver *s, int first, int n, double *c) {
+  if (!s || !c || first < 0 || n < 0) return TEUK_EINVAL;
+  if (!s->has_src || first + n > s->d.nsched) return TEUK_ERANGE;
+  if (n == 0) return TEUK_OK;
+  HIPCHK(hipStreamSynchronize(run_stream(s)));
+  HIPCHK(hipMemcpy2D(c, 12 * sizeof(double), s->d.sched + first, sizeof(SrcDev), 12 * sizeof(double), (size_t)n,
+                     hipMemcpyDeviceToHost));
+  return TEUK_OK;
+}

@@ -67,6 +67,7 @@ EXPORTS = [
     "teuk_footprint", "teuk_body", "teuk_strerror", "teuk_trajectory_schedule", "teuk_traj_plunge",
     "teuk_set_tiling", "teuk_remaining", "teuk_batch_create", "teuk_batch_destroy", "teuk_batch_add",
     "teuk_batch_remove", "teuk_batch_set_tiling", "teuk_batch_step", "teuk_batch_sync", "teuk_batch_info",
+    "teuk_source_coeffs",
 ]
 
 _lib = None
@@ -112,6 +113,7 @@ def lib():
                                                       ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "teuk_set_tiling": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "teuk_remaining": ([vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "teuk_source_coeffs": ([vp, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
         "teuk_batch_create": ([ctypes.c_int, ctypes.c_longlong, ctypes.POINTER(vp)], ctypes.c_int),
         "teuk_batch_destroy": ([vp], None),
         "teuk_batch_add": ([vp, vp], ctypes.c_int),
@@ -283,6 +285,14 @@ class Solver:
 
     def set_tiling(self, waves_per_block, waves_per_eu):
         _check(lib().teuk_set_tiling(self._h, int(waves_per_block), int(waves_per_eu)), "teuk_set_tiling")
+
+    def source_coeffs(self, first=0, n=None):
+        """Device-computed per-step source coefficients, shape (n, 12)."""
+        if n is None:
+            n = self.p.nsteps() - first
+        c = np.zeros((n, 12))
+        _check(lib().teuk_source_coeffs(self._h, int(first), int(n), _ptr(c)), "teuk_source_coeffs")
+        return c
 
     def remaining(self):
         n = ctypes.c_int()

@@ -138,6 +138,13 @@ long long teuk_footprint(const teuk_params *p, int nsteps);
  * for (2 or 4 waves per SIMD).  Results do not depend on it. */
 int teuk_set_tiling(teuk_solver *s, int waves_per_block, int waves_per_eu);
 
+/* The 12 per-step source coefficients (the canonical T_4 coefficients times
+ * e^{-i m phi_p}/pi, (0,0) (1,0) (0,1) (2,0) (1,1) (0,2) as re, im pairs)
+ * the device computed for steps first .. first+n-1, into c[n][12].  No
+ * counterpart in the reference (its sourced() keeps them in registers,
+ * kern.cu:1229-2679); for checking the source stage by stage. */
+int teuk_source_coeffs(teuk_solver *s, int first, int n, double *c);
+
 /* Steps left in the solver's schedule ((int)(T/dt), or the source schedule
  * if shorter). */
 int teuk_remaining(teuk_solver *s, int *steps);

@@ -77,7 +77,7 @@ def test_batch_members_equal_alone(integrator, wpb, wpe, order, chunk):
 
 
 def test_batch_admission_by_footprint():
-    p, _, _ = member(256, 32, -40.0, 60.0, 0.5, 2, 20, 1, False)
+    p, _, _ = member(256, 32, -40.0, 60.0, 0.5, 2, 20, 1, 1, False)
     fp = teuk.footprint(p)
     _, r_c, th = teuk.grid(p)
     with teuk.Batch(integrator=1, hbm_budget=int(2.5 * fp)) as b:
@@ -94,7 +94,7 @@ def test_batch_admission_by_footprint():
 
 
 def test_solver_in_batch_refuses_own_stepping():
-    p, _, _ = member(256, 32, -40.0, 60.0, 0.5, 2, 20, 1, False)
+    p, _, _ = member(256, 32, -40.0, 60.0, 0.5, 2, 20, 1, 1, False)
     with teuk.Batch(integrator=1) as b, teuk.Solver(p) as s:
         assert b.add(s)
         with pytest.raises(teuk.TeukError):
@@ -105,7 +105,7 @@ def test_solver_in_batch_refuses_own_stepping():
 
 
 def test_single_solver_tiling_is_bitwise_neutral():
-    p, u0, rows = member(512, 32, -40.0, 60.0, 0.6, 2, 40, 5, True)
+    p, u0, rows = member(512, 32, -40.0, 60.0, 0.6, 2, 40, 1, 5, True)
     ref = run_alone(p, u0, rows, 40)[0]
     _, r_c, th = teuk.grid(p)
     for wpb, wpe in [(1, 2), (2, 4), (4, 4)]:

@@ -77,6 +77,36 @@ def test_source_run_bitwise(a, m, integrator):
     assert np.array_equal(out["h"], ho[out["k"]])
 
 
+def oracle_coeffs(p, sched):
+    """T_4 coefficients per step from the oracle, from the same schedule."""
+    out = []
+    for st in sched:
+        state = {k: getattr(st, k) for k in ("E", "lz", "rp", "drdt", "d2rdt2", "dthdt", "d2thdt2", "dphidt",
+                                             "d2phidt2", "tp", "phip")}
+        state.update(mass=p.mass, a=p.a, nmu=1.0, mm=p.m)
+        out.append(O.t4_coeffs(state))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("case", ["interior", "edge"])
+def test_source_coefficients_bitwise(case):
+    """The device's per-step T_4 coefficients (one thread per step) against
+    the oracle's, bit for bit, for every step of the schedule."""
+    steps = 60
+    if case == "interior":
+        p = teuk.default_params(1024, 32, -50.0, 100.0, 0.9, -3, dt_over_dx=0.5)
+        rows = inspiral_rows(2 * steps + 4, p.dt)
+    else:
+        p = teuk.default_params(512, 32, -10.0, 90.0, 0.6, 2, dt_over_dx=0.5)
+        rows = inspiral_rows(2 * steps + 4, p.dt, r0=1.85, rdot=0.0, th0=0.2, thamp=0.0)
+    p.T = steps * p.dt * (1 + 1e-12)
+    sched = teuk.trajectory_schedule(rows, steps, p.dt)
+    with teuk.Solver(p) as s:
+        s.set_trajectory(rows, steps)
+        got = s.source_coeffs(0, steps)
+    assert_same_bits(got, oracle_coeffs(p, sched))
+
+
 @pytest.mark.parametrize("steps", [2, 60])   # step 0 has turn-on factor 0
 def test_patch_at_domain_edges(steps):
     """Particle close to the inner rho edge and the theta axis: the patch is

@@ -163,8 +163,8 @@ def main():
     ap.add_argument("--scale", type=float, default=0.1, help="fraction of the batch/serve solve lengths")
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="first copy 256 MiB device to device 4 times (a kernel of known traffic, for "
-                         "scaling the HBM counters of a profiler run)")
+                    help="first copy 2 GiB device to device 4 times (a kernel of known traffic, larger than "
+                         "the last-level cache, for scaling the HBM counters of a profiler run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -174,12 +174,12 @@ def main():
     from bh_ringdown_amd import teuk, workloads
     if args.copy_calibration:
         import torch
-        x = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
+        x = torch.ones(256 << 20, dtype=torch.float64, device="cuda")
         y = torch.empty_like(x)
         for _ in range(4):
             y.copy_(x)
         torch.cuda.synchronize()
-        log("bench: copy calibration, 4 x 256 MiB read + 256 MiB written")
+        log("bench: copy calibration, 4 x (2 GiB read + 2 GiB written)")
         del x, y
     integ = 1 if args.integrator == "rk4" else 0
     p = workloads.single_16384x64(integrator=integ)

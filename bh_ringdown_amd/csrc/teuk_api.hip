@@ -113,12 +113,14 @@ void flush_timing(teuk_solver *s) {
   s->ev_used = 0;
 }
 
-// One time step whose index is (device step counter + k_rel): source patch,
-// then the fused stages.  Output records are completed by launch_finalize
+// One time step whose index is (device step counter + k_rel): source patches
+// (every kRing steps, for the next kRing steps), then the fused stages.  Output records are completed by launch_finalize
 // once per batch of steps.
 int enqueue_step(teuk_solver *s, int k_rel) {
   const int nstage = s->P.integrator == 0 ? 3 : 4;
-  if (s->has_src) HIPCHK(launch_source(s->g, s->d, k_rel, s->stream));
+  // the patches of this step and the next ones (up to the ring size) come
+  // from one launch at the first step of each group of kRing steps
+  if (s->has_src && k_rel % kRing == 0) HIPCHK(launch_source(s->g, s->d, k_rel, kRing, s->stream));
   for (int st = 1; st <= nstage; st++) {
     if (s->timing) {
       if (s->ev_used + 2 > s->ev.size()) {
@@ -277,7 +279,7 @@ long long teuk_footprint(const teuk_params *p, int nsteps) {
   const long long states = (p->integrator == 0 ? 3 : 4) * 4 * NM;
   const long long nrec = (long long)(std::max(nsteps, 0) / std::max(p->Fr, 1)) + 1;
   long long bytes = 8 * (states + 2 * NM + (kNK + 1) * (long long)p->N + (kNR + 1) * (long long)p->M);
-  bytes += 8 * (2LL * p->ptsx * p->ptsy) + 8;
+  bytes += 8 * ((long long)kRing * 2 * p->ptsx * p->ptsy) + 4LL * 2 * kRing;
   bytes += 8 * (4LL * p->M + 2 * p->M + 1 + 7LL * 5 * (p->M / 4) * 3);
   bytes += nrec * (16 * 8 + 2 * 4 + 6LL * p->M * 8);
   bytes += (long long)nsteps * (long long)(sizeof(SrcDev) + sizeof(teuk_t4::StepIn));
@@ -332,8 +334,8 @@ int teuk_create(const teuk_params *p, const double *r_c, const double *theta, te
   if ((rc = dalloc(s, &d.scratch, NM))) return fail(rc);
   if ((rc = dalloc(s, &d.theta, p->M))) return fail(rc);
   if ((rc = dalloc(s, &d.r_c, p->N))) return fail(rc);
-  if ((rc = dalloc(s, &d.patch, 2 * (size_t)p->ptsx * p->ptsy))) return fail(rc);
-  if ((rc = dalloc(s, &d.patch_org, 2))) return fail(rc);
+  if ((rc = dalloc(s, &d.patch, (size_t)kRing * 2 * p->ptsx * p->ptsy))) return fail(rc);
+  if ((rc = dalloc(s, &d.patch_org, 2 * kRing))) return fail(rc);
   double *extw = nullptr, *projw = nullptr;
   if ((rc = dalloc(s, &d.ext.integ, 4 * (size_t)p->M))) return fail(rc);
   if ((rc = dalloc(s, &d.ext.snap, (size_t)g.nrec_cap * 6 * p->M))) return fail(rc);
@@ -359,7 +361,7 @@ int teuk_create(const teuk_params *p, const double *r_c, const double *theta, te
   if (hipMemsetAsync(states, 0, nstates * 4 * NM * sizeof(double), s->stream) != hipSuccess) return fail(TEUK_EHIP);
   if (hipMemsetAsync(d.ext.integ, 0, 4 * (size_t)p->M * sizeof(double), s->stream) != hipSuccess)
     return fail(TEUK_EHIP);
-  if (hipMemsetAsync(d.patch, 0, 2 * (size_t)p->ptsx * p->ptsy * sizeof(double), s->stream) != hipSuccess)
+  if (hipMemsetAsync(d.patch, 0, (size_t)kRing * 2 * p->ptsx * p->ptsy * sizeof(double), s->stream) != hipSuccess)
     return fail(TEUK_EHIP);
   Ctl c0{};
   c0.k = 0;
@@ -372,8 +374,9 @@ int teuk_create(const teuk_params *p, const double *r_c, const double *theta, te
     return fail(TEUK_EHIP);
   if (hipMemcpyAsync(d.r_c, r_c, p->N * sizeof(double), hipMemcpyHostToDevice, s->stream) != hipSuccess)
     return fail(TEUK_EHIP);
-  int org[2] = {-(1 << 29), -(1 << 29)};
-  if (hipMemcpyAsync(d.patch_org, org, sizeof(org), hipMemcpyHostToDevice, s->stream) != hipSuccess)
+  std::vector<int> org(2 * kRing, -(1 << 29));
+  if (hipMemcpyAsync(d.patch_org, org.data(), org.size() * sizeof(int), hipMemcpyHostToDevice, s->stream) !=
+      hipSuccess)
     return fail(TEUK_EHIP);
 
   // Extraction weights, body.cu:175-183 (rr is the extraction radius).
@@ -762,7 +765,8 @@ int batch_rebuild(teuk_batch *b) {
 
 int batch_enqueue_step(teuk_batch *b, int k_rel) {
   const int n = (int)b->members.size();
-  if (b->max_cells > 0) HIPCHK(launch_source_batch(b->d_desc, n, b->max_cells, k_rel, b->stream));
+  if (b->max_cells > 0 && k_rel % kRing == 0)
+    HIPCHK(launch_source_batch(b->d_desc, n, b->max_cells, k_rel, kRing, b->stream));
   const int nstage = b->integrator == 0 ? 3 : 4;
   for (int st = 1; st <= nstage; st++)
     HIPCHK(launch_stage_batch(b->d_desc, b->d_tiles, b->ntiles, b->integrator, st, k_rel, b->tiling, b->stream));
@@ -938,5 +942,19 @@ int teuk_source_coeffs(teuk_solver *s, int first, int n, double *c) {
   HIPCHK(hipStreamSynchronize(run_stream(s)));
   HIPCHK(hipMemcpy2D(c, 12 * sizeof(double), s->d.sched + first, sizeof(SrcDev), 12 * sizeof(double), (size_t)n,
                      hipMemcpyDeviceToHost));
+  return TEUK_OK;
+}
+
+int teuk_source_patch(teuk_solver *s, int k, double *patch, int *origin) {
+  if (!s || !patch || !origin || k < 0) return TEUK_EINVAL;
+  if (!s->has_src || k >= s->d.nsched) return TEUK_ERANGE;
+  hipStream_t st = run_stream(s);
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(launch_source_at(s->g, s->d, k, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const size_t cells = (size_t)s->P.ptsx * s->P.ptsy;
+  HIPCHK(hipMemcpy(patch, s->d.patch + (size_t)(k % kRing) * 2 * cells, 2 * cells * sizeof(double),
+                   hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(origin, s->d.patch_org + 2 * (k % kRing), 2 * sizeof(int), hipMemcpyDeviceToHost));
   return TEUK_OK;
 }

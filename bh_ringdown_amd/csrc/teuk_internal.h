@@ -26,6 +26,9 @@ constexpr double kD1 = 0.016666666666667;
 constexpr int kWave = 64;
 constexpr int kRows = 8;
 constexpr int kHalo = 4;            // KO reach in rho (kern.cu:247-256)
+// Source patches are prepared for up to kRing steps per launch and kept in a
+// ring of that many slots (step k uses slot k % kRing).
+constexpr int kRing = 32;
 constexpr int kLdsCols = kWave + 2 * kHalo;
 
 // Per-column constants of the coefficient fields (kern.cu:132-145 with the
@@ -100,8 +103,8 @@ struct DevState {
   double *scratch;         // [M][N] set-up scratch (bb plane)
   double *theta, *r_c;     // [M], [N]
   double *phase;           // [2][N] cos, sin of the source phase (kern.cu:2671-2672)
-  double *patch;           // [2][ptsy][ptsx] source patch
-  int *patch_org;          // [2] {row0, col0}
+  double *patch;           // [kRing][2][ptsy][ptsx] source patches
+  int *patch_org;          // [kRing][2] {row0, col0}
   SrcDev *sched;           // [nsched] source schedule, or null
   int nsched;
   Ext ext;
@@ -130,12 +133,16 @@ struct Tiling {
 // Launchers (teuk_kernels.hip).
 hipError_t launch_init_coeffs(const Grid &g, const DevState &d, hipStream_t s);
 hipError_t launch_source_coeffs(const teuk_t4::StepIn *in, SrcDev *sched, int n, hipStream_t s);
-hipError_t launch_source(const Grid &g, const DevState &d, int k_rel, hipStream_t s);
+// Patches of n <= kRing consecutive steps from (step counter + k_rel0).
+hipError_t launch_source(const Grid &g, const DevState &d, int k_rel0, int n, hipStream_t s);
+// The patch of absolute step kstep (into its ring slot).
+hipError_t launch_source_at(const Grid &g, const DevState &d, int kstep, hipStream_t s);
 hipError_t launch_stage(const Grid &g, const DevState &d, int integrator, int stage,
                         bool has_src, int k_rel, Tiling t, hipStream_t s);
 hipError_t launch_finalize(const Grid &g, const DevState &d, int n, hipStream_t s);
 // Batched forms: one launch covers every member of the batch.
-hipError_t launch_source_batch(const SolveDesc *descs, int nsolves, int max_cells, int k_rel, hipStream_t s);
+hipError_t launch_source_batch(const SolveDesc *descs, int nsolves, int max_cells, int k_rel0, int n,
+                               hipStream_t s);
 hipError_t launch_stage_batch(const SolveDesc *descs, const int2 *tiles, int ntiles, int integrator,
                               int stage, int k_rel, Tiling t, hipStream_t s);
 hipError_t launch_finalize_batch(const SolveDesc *descs, int nsolves, int n, hipStream_t s);

@@ -161,7 +161,7 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
   const double *__restrict__ ktab = d.ktab;
   const double *__restrict__ rtab = d.rtab;
   const double *__restrict__ dbdx = d.dbdx;
-  const double *__restrict__ patch = d.patch;
+  const double *__restrict__ patch = d.patch;   // advanced to this step's ring slot below
   const int *__restrict__ porg = d.patch_org;
   const Ext &ext = d.ext;
   Ctl *ctl = d.ctl;
@@ -201,17 +201,14 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
     }
   }
 
+  const int kstep = ctl->k + k_rel;
   int it0 = INT_MIN / 2, ip0 = INT_MIN / 2;
   if (has_src) {
-    it0 = porg[0];
-    ip0 = porg[1];
+    patch += (size_t)(kstep % kRing) * 2 * g.ptsx * g.ptsy;
+    it0 = porg[2 * (kstep % kRing)];
+    ip0 = porg[2 * (kstep % kRing) + 1];
   }
-  int kstep = 0;
-  bool snap = false;
-  if (kLast) {
-    kstep = ctl->k + k_rel;
-    snap = (kstep % g.Fr) == 0 && kstep / g.Fr < g.nrec_cap;
-  }
+  const bool snap = kLast && (kstep % g.Fr) == 0 && kstep / g.Fr < g.nrec_cap;
 
   double lo[4] = {0, 0, 0, 0}, hi[4] = {0, 0, 0, 0};
 
@@ -706,9 +703,11 @@ hipError_t launch_finalize_batch(const SolveDesc *descs, int nsolves, int n, hip
   return hipGetLastError();
 }
 
-hipError_t launch_source_batch(const SolveDesc *descs, int nsolves, int max_cells, int k_rel, hipStream_t s) {
-  if (nsolves <= 0 || max_cells <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_source_batch, dim3((max_cells + 255) / 256, nsolves), dim3(256), 0, s, descs, k_rel);
+hipError_t launch_source_batch(const SolveDesc *descs, int nsolves, int max_cells, int k_rel0, int n,
+                               hipStream_t s) {
+  if (nsolves <= 0 || max_cells <= 0 || n <= 0) return hipSuccess;
+  if (n > kRing) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_source_batch, dim3((max_cells + 255) / 256, nsolves, n), dim3(256), 0, s, descs, k_rel0);
   return hipGetLastError();
 }
 
@@ -717,9 +716,17 @@ hipError_t launch_finalize(const Grid &g, const DevState &d, int n, hipStream_t 
   return hipGetLastError();
 }
 
-hipError_t launch_source(const Grid &g, const DevState &d, int k_rel, hipStream_t s) {
+hipError_t launch_source(const Grid &g, const DevState &d, int k_rel0, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (n > kRing) return hipErrorInvalidValue;
   const int cells = g.ptsx * g.ptsy;
-  hipLaunchKernelGGL(k_source, dim3((cells + 255) / 256), dim3(256), 0, s, g, d, k_rel);
+  hipLaunchKernelGGL(k_source, dim3((cells + 255) / 256, n), dim3(256), 0, s, g, d, k_rel0);
+  return hipGetLastError();
+}
+
+hipError_t launch_source_at(const Grid &g, const DevState &d, int kstep, hipStream_t s) {
+  const int cells = g.ptsx * g.ptsy;
+  hipLaunchKernelGGL(k_source_at, dim3((cells + 255) / 256), dim3(256), 0, s, g, d, kstep);
   return hipGetLastError();
 }
 

@@ -67,7 +67,7 @@ EXPORTS = [
     "teuk_footprint", "teuk_body", "teuk_strerror", "teuk_trajectory_schedule", "teuk_traj_plunge",
     "teuk_set_tiling", "teuk_remaining", "teuk_batch_create", "teuk_batch_destroy", "teuk_batch_add",
     "teuk_batch_remove", "teuk_batch_set_tiling", "teuk_batch_step", "teuk_batch_sync", "teuk_batch_info",
-    "teuk_source_coeffs",
+    "teuk_source_coeffs", "teuk_source_patch",
 ]
 
 _lib = None
@@ -114,6 +114,7 @@ def lib():
         "teuk_set_tiling": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "teuk_remaining": ([vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "teuk_source_coeffs": ([vp, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
+        "teuk_source_patch": ([vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "teuk_batch_create": ([ctypes.c_int, ctypes.c_longlong, ctypes.POINTER(vp)], ctypes.c_int),
         "teuk_batch_destroy": ([vp], None),
         "teuk_batch_add": ([vp, vp], ctypes.c_int),
@@ -293,6 +294,13 @@ class Solver:
         c = np.zeros((n, 12))
         _check(lib().teuk_source_coeffs(self._h, int(first), int(n), _ptr(c)), "teuk_source_coeffs")
         return c
+
+    def source_patch(self, k):
+        """Device-computed source patch of step k: (patch[2, ptsy, ptsx], (row0, col0))."""
+        buf = np.zeros((2, self.p.ptsy, self.p.ptsx))
+        org = (ctypes.c_int * 2)()
+        _check(lib().teuk_source_patch(self._h, int(k), _ptr(buf), org), "teuk_source_patch")
+        return buf, (org[0], org[1])
 
     def remaining(self):
         n = ctypes.c_int()

@@ -145,6 +145,12 @@ int teuk_set_tiling(teuk_solver *s, int waves_per_block, int waves_per_eu);
  * kern.cu:1229-2679); for checking the source stage by stage. */
 int teuk_source_coeffs(teuk_solver *s, int first, int n, double *c);
 
+/* The source patch of step k (2 x ptsy x ptsx: real plane, imaginary
+ * plane; cell (j, i) at grid point (origin[0] + j, origin[1] + i)), as the
+ * device computes it for that step (kern.cu:770-2683).  For checking; call
+ * it while the solver is idle. */
+int teuk_source_patch(teuk_solver *s, int k, double *patch, int *origin);
+
 /* Steps left in the solver's schedule ((int)(T/dt), or the source schedule
  * if shorter). */
 int teuk_remaining(teuk_solver *s, int *steps);

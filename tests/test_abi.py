@@ -64,8 +64,13 @@ def test_create_rejects_bad_arguments_before_touching_the_gpu():
 def test_footprint_scales_with_grid():
     p1 = teuk.default_params(4096, 32, -400.0, 1200.0, 0.5, 2, T=100.0)
     p2 = teuk.default_params(8192, 64, -400.0, 1200.0, 0.5, 2, T=100.0)
+    for p in (p1, p2):
+        p.ptsx = p.ptsy = 8          # keep the patch ring small beside the grid
     f1, f2 = teuk.footprint(p1), teuk.footprint(p2)
     assert 3.5 < f2 / f1 < 4.5
+    # the source-patch ring: 32 slots of 2 x ptsx x ptsy doubles
+    p3 = teuk.default_params(4096, 32, -400.0, 1200.0, 0.5, 2, T=100.0)
+    assert teuk.footprint(p3) - f1 == 32 * 2 * (128 * 128 - 8 * 8) * 8
     # 3 states x 4 fields + the dbdx plane + set-up scratch, 8 B each
     assert f1 >= 8 * (12 + 2) * 4096 * 32
 

@@ -107,6 +107,37 @@ def test_source_coefficients_bitwise(case):
     assert_same_bits(got, oracle_coeffs(p, sched))
 
 
+@pytest.mark.parametrize("case", ["interior", "edge"])
+def test_source_patch_bitwise(case):
+    """The device's source patch (Gaussian deltas, post-factors, clipping)
+    against the oracle's source array at several steps, bit for bit."""
+    steps = 60
+    if case == "interior":
+        p = teuk.default_params(1024, 32, -50.0, 100.0, 0.9, -3, dt_over_dx=0.5)
+        rows = inspiral_rows(2 * steps + 4, p.dt)
+    else:
+        p = teuk.default_params(512, 32, -10.0, 90.0, 0.6, 2, dt_over_dx=0.5)
+        rows = inspiral_rows(2 * steps + 4, p.dt, r0=1.85, rdot=0.0, th0=0.2, thamp=0.0)
+    p.T = steps * p.dt * (1 + 1e-12)
+    _, r_c, th = teuk.grid(p)
+    sched = teuk.trajectory_schedule(rows, steps, p.dt)
+    op = O.make_params(p.N, p.M, p.xmin, p.xmax, p.a, p.m, dt=p.dt)
+    with teuk.Solver(p, r_c, th) as s:
+        s.set_trajectory(rows, steps)
+        for k in (1, 2, 17, 40, 59):
+            patch, (r0, c0) = s.source_patch(k)
+            st = O.SrcStep(*[getattr(sched[k], f) for f, _ in teuk.SourceStep._fields_])
+            full = O.source(op, th, r_c, st)
+            want = np.zeros_like(patch)
+            rr = np.arange(p.ptsy) + r0
+            cc = np.arange(p.ptsx) + c0
+            ok_r = (rr >= 0) & (rr < p.M)
+            ok_c = (cc >= 0) & (cc < p.N)
+            want[:, np.ix_(ok_r, ok_c)[0], np.ix_(ok_r, ok_c)[1]] = full[:, rr[ok_r]][:, :, cc[ok_c]]
+            print(f"patch case={case} step={k}", flush=True)
+            assert_same_bits(patch, want)
+
+
 @pytest.mark.parametrize("steps", [2, 60])   # step 0 has turn-on factor 0
 def test_patch_at_domain_edges(steps):
     """Particle close to the inner rho edge and the theta axis: the patch is

@@ -32,16 +32,17 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 # SSP-RK3 64 + 96 + 96; classic RK4 in accumulator form 96 + 160 + 160 + 96.
 B_ALG = {"rk3": 256.0, "rk4": 512.0}
 STAGES = {"rk3": 3, "rk4": 4}
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_stage_pmc_16384x64.txt")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_stage_rk4_16384x64_plunge_pmc.txt")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def pmc_traffic():
-    """HBM bytes per stage launch from the committed rocprofv3 --pmc summary."""
-    if not os.path.exists(PMC_SUMMARY):
+def pmc_traffic(integrator):
+    """HBM bytes per stage launch (FETCH_SIZE + WRITE_SIZE, scaled by the copy
+    calibration) from the committed rocprofv3 --pmc summary of the RK4 run."""
+    if integrator != "rk4" or not os.path.exists(PMC_SUMMARY):
         return None
     m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(PMC_SUMMARY).read(), re.M)
     return float(m.group(1)) if m else None
@@ -211,7 +212,7 @@ def main():
     # B_alg is per full step; a step is STAGES stage launches
     b_alg = B_ALG[args.integrator]
     achieved = (b_alg / STAGES[args.integrator]) * pts / per_launch_s / 1e9
-    traffic = pmc_traffic()
+    traffic = pmc_traffic(args.integrator)
     out = {
         "metric": "FP64 point-updates/s (%HBM roofline)",
         "value": value,

@@ -107,40 +107,41 @@ void flush_timing(teuk_solver *s) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, s->ev[i], s->ev[i + 1]) == hipSuccess) {
       s->timed_ms += ms;
-      s->timed_launches += 1;
+      s->timed_launches += s->P.integrator == 0 ? 3 : 4;
     }
   }
   s->ev_used = 0;
 }
 
 // One time step whose index is (device step counter + k_rel): source patches
-// (every kRing steps, for the next kRing steps), then the fused stages.  Output records are completed by launch_finalize
-// once per batch of steps.
+// (every kRing steps, for the next kRing steps), then the fused stages.
+// Output records are completed by launch_finalize once per batch of steps.
 int enqueue_step(teuk_solver *s, int k_rel) {
   const int nstage = s->P.integrator == 0 ? 3 : 4;
   // the patches of this step and the next ones (up to the ring size) come
   // from one launch at the first step of each group of kRing steps
   if (s->has_src && k_rel % kRing == 0) HIPCHK(launch_source(s->g, s->d, k_rel, kRing, s->stream));
-  for (int st = 1; st <= nstage; st++) {
-    if (s->timing) {
-      if (s->ev_used + 2 > s->ev.size()) {
-        if (s->ev.size() < 4096) {
-          for (int i = 0; i < 64; i++) {
-            hipEvent_t e;
-            HIPCHK(hipEventCreate(&e));
-            s->ev.push_back(e);
-          }
-        } else {
-          flush_timing(s);
+  // Timing: one event pair around the step's back-to-back stage launches
+  // (an event between two launches would add its own gap to both).
+  if (s->timing) {
+    if (s->ev_used + 2 > s->ev.size()) {
+      if (s->ev.size() < 4096) {
+        for (int i = 0; i < 64; i++) {
+          hipEvent_t e;
+          HIPCHK(hipEventCreate(&e));
+          s->ev.push_back(e);
         }
+      } else {
+        flush_timing(s);
       }
-      HIPCHK(hipEventRecord(s->ev[s->ev_used], s->stream));
-      HIPCHK(launch_stage(s->g, s->d, s->P.integrator, st, s->has_src, k_rel, s->tiling, s->stream));
-      HIPCHK(hipEventRecord(s->ev[s->ev_used + 1], s->stream));
-      s->ev_used += 2;
-    } else {
-      HIPCHK(launch_stage(s->g, s->d, s->P.integrator, st, s->has_src, k_rel, s->tiling, s->stream));
     }
+    HIPCHK(hipEventRecord(s->ev[s->ev_used], s->stream));
+  }
+  for (int st = 1; st <= nstage; st++)
+    HIPCHK(launch_stage(s->g, s->d, s->P.integrator, st, s->has_src, k_rel, s->tiling, s->stream));
+  if (s->timing) {
+    HIPCHK(hipEventRecord(s->ev[s->ev_used + 1], s->stream));
+    s->ev_used += 2;
   }
   return TEUK_OK;
 }

@@ -123,9 +123,10 @@ int teuk_read_output(teuk_solver *s, int first, int max, teuk_output_row *rows, 
  * recorded with each output row: psi[(row - first)][2][M]. */
 int teuk_read_psi(teuk_solver *s, int first, int max, double *psi, int *n);
 
-/* Kernel timing: when enabled, teuk_step brackets every stage-kernel launch
- * with HIP events on the solver stream.  teuk_timing returns the number of
- * timed stage launches and their summed duration in milliseconds. */
+/* Kernel timing: when enabled, teuk_step brackets each step's back-to-back
+ * stage-kernel launches with a pair of HIP events on the solver stream.
+ * teuk_timing returns the number of timed stage launches and their summed
+ * duration in milliseconds. */
 int teuk_enable_timing(teuk_solver *s, int on);
 int teuk_timing(teuk_solver *s, long long *launches, double *ms);
 

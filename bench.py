@@ -81,11 +81,12 @@ def cpu_baseline(p, rows, seconds=12.0):
                       f"{p.N}x{p.M} plunge solve; oracle g++ -O3 OpenMP, set-up excluded"}
 
 
-def batch_section(scale, log_every=True):
-    """The 72 uneven solves (9 m x 8 spins, grids by spin) at `scale` of their
-    final time: the plain one-solve-at-a-time loop against one batch."""
+def batch_section(scale, grid=None):
+    """The 72 uneven solves (9 m x 8 spins, grids by spin, or all on `grid`)
+    at `scale` of their final time: the plain one-solve-at-a-time loop
+    against one batch."""
     from bh_ringdown_amd import teuk, tune, workloads
-    items = workloads.batch_72(seed=0, t_final=1200.0 * scale)
+    items = workloads.batch_72(seed=0, t_final=1200.0 * scale, grid=grid)
     total = sum(p.N * p.M * p.nsteps() for p, _ in items)
     log(f"bench: batch of {len(items)} solves, {total:.3e} point-updates")
     probe = {0, 31, 71}
@@ -118,8 +119,9 @@ def batch_section(scale, log_every=True):
         s.close()
     t_batch = time.perf_counter() - t0
     log(f"bench: batched {t_batch:.2f} s")
-    return {"workload": f"72 solves (m -4..4 x 8 seeded spins; N x M by spin in 4096x32, 8192x32, "
-                        f"16384x64) to t_final {1200.0 * scale:g} ({scale:g} of 1200), RK4, plunge source",
+    grids = f"all {grid[0]}x{grid[1]}" if grid else "N x M by spin in 4096x32, 8192x32, 16384x64"
+    return {"workload": f"72 solves (m -4..4 x 8 seeded spins; {grids}) to t_final {1200.0 * scale:g} "
+                        f"({scale:g} of 1200), RK4, plunge source",
             "solves_per_hour": 3600.0 * len(items) / t_batch, "plain_solves_per_hour": 3600.0 * len(items) / t_plain,
             "speedup_vs_plain": t_plain / t_batch, "point_updates_per_s": total / t_batch,
             "plain_point_updates_per_s": total / t_plain, "tiling": tiling, "members_bitwise_equal_alone": same}
@@ -239,6 +241,7 @@ def main():
     }
     if not args.no_batch:
         out["batch"] = batch_section(args.scale)
+        out["batch_small_grids"] = batch_section(args.scale, grid=(4096, 32))
     if not args.no_serve:
         out["serving"] = serve_section(args.scale, args.serve_seconds)
     if not args.no_cpu:

@@ -72,17 +72,18 @@ def grid_for_spin(a):
     return 16384, 64
 
 
-def batch_72(seed=0, t_final=1200.0, integrator=1, eta=1e-3, iota_deg=30.0):
+def batch_72(seed=0, t_final=1200.0, integrator=1, eta=1e-3, iota_deg=30.0, grid=None):
     """Batched uneven solves: m in -4..4 crossed with 8 spins a ~ U[0, 0.99)
     drawn from the seed, each spin's grid from grid_for_spin() on rho in
     [-400, 1200], dt = 0.5 drho, to t_final, all with the same inclined
     plunge (mass ratio eta, inclination iota) in each spin's geometry.
-    Returns [(params, trajectory rows)] (72 entries)."""
+    grid=(N, M) gives every solve that grid instead.  Returns
+    [(params, trajectory rows)] (72 entries)."""
     rng = np.random.default_rng(seed)
     spins = rng.uniform(0.0, 0.99, size=8)
     out = []
     for a in spins:
-        N, M = grid_for_spin(a)
+        N, M = grid if grid else grid_for_spin(a)
         rows = None
         for m in range(-4, 5):
             p = teuk.default_params(N, M, -400.0, 1200.0, float(a), m, dt_over_dx=0.5, T=t_final)

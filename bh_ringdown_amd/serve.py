@@ -100,14 +100,22 @@ def prepare(req, integrator=1):
 
 
 class Server:
-    """Continuous batching over a libteuk batch with an HBM budget."""
+    """Continuous batching over a libteuk batch with an HBM budget.
 
-    def __init__(self, hbm_budget, chunk=64, integrator=1, tiling=None, max_resident=64):
+    Admission is first come, first served, limited by the HBM budget and by
+    an occupancy target: once the resident solves together update
+    `target_points` grid points per step (enough wavefronts to keep every
+    compute unit streaming), admitting more would not raise throughput and
+    would only stretch every resident solve's latency, so later requests
+    wait for a resident one to finish."""
+
+    def __init__(self, hbm_budget, chunk=64, integrator=1, tiling=None, max_resident=64, target_points=3 << 20):
         self.hbm_budget = int(hbm_budget)
         self.chunk = chunk
         self.integrator = integrator
         self.tiling = tiling or {}
         self.max_resident = max_resident
+        self.target_points = int(target_points)
 
     def run(self, requests, clock=time.perf_counter, on_records=None):
         """Serve `requests` (sorted by arrival); returns {id: Result}."""
@@ -131,6 +139,9 @@ class Server:
                 while pending and pending[0].arrival <= now():
                     queue.append(pending.pop(0))
                 while queue and len(live) < self.max_resident:
+                    resident = sum(v[1].N * v[1].M for v in live.values())
+                    if live and resident >= self.target_points:
+                        break
                     req = queue[0]
                     p = request_params(req, self.integrator)
                     if not live and teuk.footprint(p) > self.hbm_budget:

@@ -33,9 +33,10 @@ def gaussian_pulse(p, seed, center=50.0, width=5.0):
     return u
 
 
-def plain_1024x32(steps=2000, integrator=0, Fr=1):
-    """Host-oracle parity case: s=-2, m=2, a=0.5; N=1024 on rho in [-100,300],
-    M=32; dt = 0.5 dx; no source; Gaussian initial Psi; Psi at rho=250."""
+def plain_1024x32(steps=2000, integrator=1, Fr=1):
+    """Host-oracle parity case (the plain method): s=-2, m=2, a=0.5; N=1024
+    on rho in [-100,300], M=32; classic RK4 (integrator=0: the reference's
+    SSP-RK3); dt = 0.5 dx; no source; Gaussian initial Psi; Psi at rho=250."""
     p = teuk.default_params(1024, 32, -100.0, 300.0, 0.5, 2, dt_over_dx=0.5)
     p.T = steps * p.dt * (1.0 + 1e-12)
     p.Fr = Fr

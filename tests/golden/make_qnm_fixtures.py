@@ -58,6 +58,33 @@ def main():
         out[f"table_{i}"] = tables[name]
     np.savez_compressed(os.path.join(HERE, "qnm_a07.npz"), **out)
     print("wrote", os.path.join(HERE, "qnm_a07.npz"), len(tables), "tables")
+    # table rows for fitting the a = 0.5, m = 2 plain-method waveform
+    write_tables(0.5, [2], os.path.join(HERE, "qnm_tables_a05_m2.npz"))
+
+
+def table_names(a, modes):
+    names = set()
+    for m in modes:
+        for lv in qnm.mode_list(m, MMAX):
+            for lp, mp in qnm.spheroidal_modes(m, MMAX):
+                names.add(f"swsh/l{lv}m{mp}lp{abs(lp)}np0.dat" if mp >= 0 else f"swsh/l{lv}mm{-mp}lp{abs(lp)}np0.dat")
+                mm_, aa = mp, a
+                while mm_ < 0:
+                    aa, mm_ = -aa, -mm_
+                names.add(f"kerr/n1l{abs(lp)}m{mm_}.dat" if aa >= 0 else f"kerr/n1l{abs(lp)}mm{mm_}.dat")
+    return sorted(names)
+
+
+def write_tables(a, modes, path):
+    out = {}
+    names = table_names(a, modes)
+    for i, name in enumerate(names):
+        d = np.atleast_2d(np.loadtxt(os.path.join(REF, "Tables", name)))
+        keep = np.abs(d[:, 0] - a) <= (2e-4 if name.startswith("kerr") else 0.06)
+        out[f"table_{i}"] = d[keep][:, :3]
+    out["table_names"] = np.array(names)
+    np.savez_compressed(path, **out)
+    print("wrote", path, len(names), "tables")
 
 
 if __name__ == "__main__":

@@ -16,6 +16,7 @@ pytestmark = pytest.mark.gpu
 
 FIELD_TOL = 1e-11
 WAVE_TOL = 1e-8
+QNM_TOL = 1e-8      # relative, on the fitted QNM amplitudes and phases
 
 
 def rel(a, b):
@@ -74,9 +75,26 @@ def check(p, steps, seed=3):
     return e
 
 
-def test_plain_1024x32_full_run():
-    """The plain-method case end to end: 2000 SSP-RK3 steps, Psi at rho=250."""
-    p = workloads.plain_1024x32(steps=2000)
+def qnm_amplitudes(out, p):
+    """Stable-window QNM amplitudes (|C|, phase) of an h_lm record set, the
+    fit anchored at the peak of |h_22|."""
+    import os
+    from bh_ringdown_amd import qnm
+    fix = np.load(os.path.join(os.path.dirname(__file__), "golden", "qnm_tables_a05_m2.npz"))
+    names = [str(n) for n in fix["table_names"]]
+    tables = qnm.QnmTables(arrays={n: fix[f"table_{i}"] for i, n in enumerate(names)})
+    wave = np.column_stack([out["t"], out["h"]])
+    t_peak = float(out["t"][np.argmax(np.hypot(out["h"][:, 0], out["h"][:, 1]))])
+    ts, C = qnm.fit_amplitudes(wave, int(p.m), p.a, 4, t_peak, tables)
+    amp, phase, _ = qnm.stable_window(ts, C)
+    return amp, phase
+
+
+@pytest.mark.parametrize("integrator", [1, 0])
+def test_plain_1024x32_full_run(integrator):
+    """The plain-method case end to end: 2000 steps (RK4, and the reference's
+    SSP-RK3), Psi at rho=250; for RK4 also the QNM amplitudes fitted to h_lm."""
+    p = workloads.plain_1024x32(steps=2000, integrator=integrator)
     u0 = workloads.gaussian_pulse(p, seed=1)
     ug, out, psig = run_gpu(p, u0, 2000)
     uo, psio, ho, dedto = run_oracle(p, u0, 2000)
@@ -93,6 +111,11 @@ def test_plain_1024x32_full_run():
     for k in range(2000):
         assert out["t"][k] == t
         t += p.dt
+    if integrator == 1:
+        amp_g, ph_g = qnm_amplitudes(out, p)
+        amp_o, ph_o = qnm_amplitudes({"t": out["t"], "h": ho}, p)
+        assert np.all(np.abs(amp_g - amp_o) <= QNM_TOL * np.abs(amp_o))
+        assert np.all(np.abs(ph_g - ph_o) <= QNM_TOL * np.maximum(1.0, np.abs(ph_o)))
 
 
 @pytest.mark.parametrize("N,M,xmin,xmax,cfl", [

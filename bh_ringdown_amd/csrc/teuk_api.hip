@@ -279,7 +279,7 @@ long long teuk_footprint(const teuk_params *p, int nsteps) {
   const long long NM = (long long)p->N * p->M;
   const long long states = (p->integrator == 0 ? 3 : 4) * 4 * NM;
   const long long nrec = (long long)(std::max(nsteps, 0) / std::max(p->Fr, 1)) + 1;
-  long long bytes = 8 * (states + (kNK + 1) * (long long)p->N + (kNR + 1) * (long long)p->M);
+  long long bytes = 8 * (states + 2 * NM + (kNK + 1) * (long long)p->N + (kNR + 1) * (long long)p->M);
   bytes += 8 * ((long long)kRing * 2 * p->ptsx * p->ptsy) + 4LL * 2 * kRing;
   bytes += 8 * (4LL * p->M + 2 * p->M + 1 + 7LL * 5 * (p->M / 4) * 3);
   bytes += nrec * (16 * 8 + 2 * 4 + 6LL * p->M * 8);
@@ -331,6 +331,8 @@ int teuk_create(const teuk_params *p, const double *r_c, const double *theta, te
   d.A = nstates == 4 ? states + 12 * NM : nullptr;
   if ((rc = dalloc(s, &d.ktab, kNK * (size_t)p->N))) return fail(rc);
   if ((rc = dalloc(s, &d.rtab, kNR * (size_t)p->M))) return fail(rc);
+  if ((rc = dalloc(s, &d.dbdx, NM))) return fail(rc);
+  if ((rc = dalloc(s, &d.scratch, NM))) return fail(rc);
   if ((rc = dalloc(s, &d.theta, p->M))) return fail(rc);
   if ((rc = dalloc(s, &d.r_c, p->N))) return fail(rc);
   if ((rc = dalloc(s, &d.patch, (size_t)kRing * 2 * p->ptsx * p->ptsy))) return fail(rc);
@@ -401,6 +403,7 @@ int teuk_create(const teuk_params *p, const double *r_c, const double *theta, te
       hipMemcpyAsync(projw, h_projw.data(), h_projw.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) !=
           hipSuccess)
     return fail(TEUK_EHIP);
+  if (launch_init_coeffs(g, d, s->stream) != hipSuccess) return fail(TEUK_EHIP);
   if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(TEUK_EHIP);
   *out = s;
   return TEUK_OK;

@@ -99,6 +99,8 @@ struct DevState {
   double *U, *C, *D, *A;   // field states (4 planes each); A only for RK4
   double *ktab;            // [kNK][N] column constants
   double *rtab;            // [kNR][M] row values
+  double *dbdx;            // [M][N] rho derivative of bb (kern.cu:192-196)
+  double *scratch;         // [M][N] set-up scratch (bb plane)
   double *theta, *r_c;     // [M], [N]
   double *phase;           // [2][N] cos, sin of the source phase (kern.cu:2671-2672)
   double *patch;           // [kRing][2][ptsy][ptsx] source patches
@@ -129,6 +131,7 @@ struct Tiling {
 };
 
 // Launchers (teuk_kernels.hip).
+hipError_t launch_init_coeffs(const Grid &g, const DevState &d, hipStream_t s);
 hipError_t launch_source_coeffs(const teuk_t4::StepIn *in, SrcDev *sched, int n, hipStream_t s);
 // Patches of n <= kRing consecutive steps from (step counter + k_rel0).
 hipError_t launch_source(const Grid &g, const DevState &d, int k_rel0, int n, hipStream_t s);

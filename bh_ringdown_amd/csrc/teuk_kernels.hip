@@ -10,11 +10,10 @@
 // times per step at body.cu:131-165).
 //
 // Data per point and stage: the stage input (4 doubles, read once), the step
-// base and the RK4 accumulator where the stage needs them (4 doubles each)
-// and the output (4 doubles).  The nine coefficient fields of kernel_init
-// are not stored: each is rebuilt per point from 12 per-column and 4 per-row
-// values with one division and one square root, and the rho derivative of
-// bb from the neighbouring columns' bb through LDS.
+// base for stages 2-3 (4 doubles), the output (4 doubles) and the
+// time-independent dbdx plane (1 double).  The nine coefficient fields of
+// kernel_init are not stored: each is rebuilt per point from 13 per-column and
+// 3 per-row values with one division and one square root.
 //
 // Tile: a wavefront owns 64 consecutive rho columns (coalesced, rho is the
 // contiguous index of parm.h:13) and a chunk of 8 theta rows.  Psi's theta
@@ -51,15 +50,6 @@ struct Coef {
   double bb, cc, ee, are, aim, dre, dim, vre, vim;
 };
 
-// bb alone (kern.cu:147) from the four column constants it needs; the same
-// operations as in coef_point, so the two agree bit for bit.
-__device__ __forceinline__ double bb_point(double K1, double K2, double KA, double KR, double sn2) {
-  const double bir = K2 * sn2 - K1;
-  const double ib = 1.0 / bir;
-  const double Atr = KA * ib, Arr = KR * ib;
-  return -0.5 * (Atr + __dsqrt_rn(Atr * Atr + 4.0 * Arr));
-}
-
 __device__ __forceinline__ Coef coef_point(const double (&K)[kNK], double sn2, double cs, double Y) {
   const double bir = K[kK2] * sn2 - K[kK1];
   const double ib = 1.0 / bir;
@@ -78,6 +68,39 @@ __device__ __forceinline__ Coef coef_point(const double (&K)[kNK], double sn2, d
   c.vre = -CRe;
   c.vim = -(K[kKCi] * ib);
   return c;
+}
+
+// ---------------------------------------------------------------------------
+// bb on interior rows (kern.cu:147) into a scratch plane, then its 6th-order
+// rho derivative (kern.cu:192-196), which is time independent.
+__global__ __launch_bounds__(256) void k_init_bb(Grid g, const double *__restrict__ ktab,
+                                                 const double *__restrict__ rtab, double *__restrict__ bbp) {
+  const size_t NM = (size_t)g.N * g.M;
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= NM) return;
+  const int k = (int)(id / g.N), j = (int)(id - (size_t)k * g.N);
+  double bb = 0.0;
+  if (k >= 3 && k < g.M - 3) {
+    double K[kNK];
+#pragma unroll
+    for (int i = 0; i < kNK; i++) K[i] = ktab[(size_t)i * g.N + j];
+    bb = coef_point(K, rtab[k], rtab[g.M + k], rtab[2 * g.M + k]).bb;
+  }
+  bbp[id] = bb;
+}
+
+__global__ __launch_bounds__(256) void k_init_dbdx(Grid g, const double *__restrict__ bbp,
+                                                   double *__restrict__ dbdx) {
+  const size_t NM = (size_t)g.N * g.M;
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= NM) return;
+  const int a = (int)(id % g.N);
+  double d = 0.0;
+  if (a >= 3 && a < g.N - 3)
+    d = (45.0 * bbp[id + 1] - 45.0 * bbp[id - 1] - 9.0 * bbp[id + 2] + 9.0 * bbp[id - 2] + bbp[id + 3] -
+         bbp[id - 3]) *
+        (kD1 * (1.0 / g.dx));
+  dbdx[id] = d;
 }
 
 // ---------------------------------------------------------------------------
@@ -124,7 +147,7 @@ template <int INTEG, int STAGE>
 __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int has_src, int k_rel, int strip,
                                            int grp, int wpb) {
   constexpr bool kLast = (INTEG == 0 && STAGE == 3) || (INTEG == 1 && STAGE == 4);
-  __shared__ double lds[4][5][kLdsCols];   // per wavefront: q_re, q_im, p_re, p_im, bb
+  __shared__ double lds[4][4][kLdsCols];
   const int N = g.N, M = g.M;
   const size_t NM = (size_t)N * M;
   // The wavefront index is uniform: make that visible to the compiler so the
@@ -137,6 +160,7 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
   double *acc = io.acc;
   const double *__restrict__ ktab = d.ktab;
   const double *__restrict__ rtab = d.rtab;
+  const double *__restrict__ dbdx = d.dbdx;
   const double *__restrict__ patch = d.patch;   // advanced to this step's ring slot below
   const int *__restrict__ porg = d.patch_org;
   const Ext &ext = d.ext;
@@ -225,14 +249,14 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
 
   // Global inputs of a row.  The stencil inputs (p and the halo columns) are
   // fetched one row ahead; the inputs used only after the LDS stencils
-  // (stage base, RK4 accumulator, source patch, row values) are issued at the top
+  // (stage base, RK4 accumulator, dbdx, source patch, row values) are issued at the top
   // of their own row, before the next row's prefetch, so that waiting for
   // them never waits for the prefetch (loads retire in issue order).
   struct RowIn {
     double pr, pi, halo[4];
   };
   struct RowLate {
-    double base[4], acc[4], sr, si, rt[kNR];  // rt: the row's sin^2, cos, Y, cot
+    double base[4], acc[4], dbdx, sr, si, rt[kNR];  // rt: the row's sin^2, cos, Y, cot
   };
   const int slot = lane < kHalo ? lane : kWave + lane;  // halo slot of lanes 0..7
   const int hcol = a0 - kHalo + slot;
@@ -245,11 +269,6 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
   // ... and every lane writes its halo value to LDS: a lane without a halo
   // column writes its own column's value (the same bits) to its own slot.
   const int hslot = halo_lane ? slot : lane + kHalo;
-  // The rho derivative of bb (kern.cu:192-196) is formed per row from bb in
-  // LDS: each lane's own bb comes with its coefficients, a halo lane also
-  // builds bb for its halo column from that column's four constants.
-  const double hK1 = ktab[(size_t)kK1 * N + hsafe], hK2 = ktab[(size_t)kK2 * N + hsafe];
-  const double hKA = ktab[(size_t)kKA * N + hsafe], hKR = ktab[(size_t)kKR * N + hsafe];
   constexpr bool kBase = (INTEG == 0 && STAGE >= 2) || (INTEG == 1 && (STAGE == 2 || STAGE == 3));
   constexpr bool kAccIn = INTEG == 1 && STAGE >= 2;
   auto load_row = [&](int b) {
@@ -268,6 +287,7 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
     for (int f = 0; f < 4; f++) x.base[f] = kBase ? base[f * NM + c] : 0.0;
 #pragma unroll
     for (int f = 0; f < 4; f++) x.acc[f] = kAccIn ? acc[f * NM + c] : 0.0;
+    x.dbdx = dbdx[c];
 #pragma unroll
     for (int j = 0; j < kNR; j++) x.rt[j] = rtab[j * M + b];
     const int sb = b - it0, sa = a - ip0;
@@ -302,15 +322,6 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
     L[3 * kLdsCols + lane + kHalo] = pi0;
 #pragma unroll
     for (int f = 0; f < 4; f++) L[f * kLdsCols + hslot] = cur.halo[f];
-    wave_sync();
-
-    // Coefficients of this point (every lane: the bb row feeds the dbdx
-    // stencil of the neighbours), bb of the halo columns, and the bb row in LDS.
-    const Coef cf = coef_point(K, late.rt[0], late.rt[1], late.rt[2]);
-    const double bbh = bb_point(hK1, hK2, hKA, hKR, late.rt[0]);
-    double *Lb = L + 4 * kLdsCols;
-    Lb[lane + kHalo] = cf.bb;
-    Lb[hslot] = halo_lane ? bbh : cf.bb;   // a lane without a halo column rewrites its own value
     wave_sync();
 
     double v[4] = {0, 0, 0, 0};
@@ -348,9 +359,8 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
       dpi = 0.1 * dpi * 0.0625 * over_dt;
 
       const size_t id = c0 + a;
-      const double dbdx = (45.0 * Lb[c + 1] - 45.0 * Lb[c - 1] - 9.0 * Lb[c + 2] + 9.0 * Lb[c - 2] + Lb[c + 3] -
-                           Lb[c - 3]) * wx;
-      const double dre = cf.dre - dbdx * cf.ee;
+      const Coef cf = coef_point(K, late.rt[0], late.rt[1], late.rt[2]);
+      const double dre = cf.dre - late.dbdx * cf.ee;
       const double sr = late.sr, si = late.si;
 
       double r[4];
@@ -598,6 +608,16 @@ bool source_available() { return true; }
 
 // ---------------------------------------------------------------------------
 // Launchers.
+hipError_t launch_init_coeffs(const Grid &g, const DevState &d, hipStream_t s) {
+  const size_t NM = (size_t)g.N * g.M;
+  const unsigned nb = (unsigned)((NM + 255) / 256);
+  hipLaunchKernelGGL(k_init_bb, dim3(nb), dim3(256), 0, s, g, d.ktab, d.rtab, d.scratch);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_init_dbdx, dim3(nb), dim3(256), 0, s, g, d.scratch, d.dbdx);
+  return hipGetLastError();
+}
+
 Tiling default_tiling(int M) {
   const int chunks = M / kRows;
   Tiling t;

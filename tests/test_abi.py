@@ -71,8 +71,8 @@ def test_footprint_scales_with_grid():
     # the source-patch ring: 32 slots of 2 x ptsx x ptsy doubles
     p3 = teuk.default_params(4096, 32, -400.0, 1200.0, 0.5, 2, T=100.0)
     assert teuk.footprint(p3) - f1 == 32 * 2 * (128 * 128 - 8 * 8) * 8
-    # 3 states x 4 fields, 8 B each
-    assert f1 >= 8 * 12 * 4096 * 32
+    # 3 states x 4 fields + the dbdx plane + set-up scratch, 8 B each
+    assert f1 >= 8 * (12 + 2) * 4096 * 32
 
 
 def synthetic_traj(n, dt, t0=0.0):

@@ -8,6 +8,7 @@
 #include <cstdint>
 
 #define TEUK_HD __host__ __device__ inline
+#include "teuk_point.h"
 #include "teuk_t4.h"
 
 namespace teuk {
@@ -31,12 +32,20 @@ constexpr int kHalo = 4;            // KO reach in rho (kern.cu:247-256)
 constexpr int kRing = 32;
 constexpr int kLdsCols = kWave + 2 * kHalo;
 
-// Per-column constants of the coefficient fields (kern.cu:132-145 with the
-// theta dependence factored out).  With bir = K2 sin^2 - K1 and ib = 1/bir:
-//   A^tr = KA ib, A^rr = KR ib, A^yy = -KD ib, B^tau_re = KBt ib,
-//   B^rho_re = KBr ib, C_re = (KC0 + KD Y) ib, B^tau_im = (KI0 - KI1 cos) ib,
-//   B^rho_im = KBi ib, C_im = KCi ib,  Y = 4 cot^2 - 4 m cot csc + m^2 csc^2.
-enum { kK1, kK2, kKA, kKR, kKD, kKBt, kKBr, kKC0, kKI0, kKI1, kKBi, kKCi, kNK };
+// Per-column constants of the coefficient fields (teuk_point.h).
+using teuk_pt::kK1;
+using teuk_pt::kK2;
+using teuk_pt::kKA;
+using teuk_pt::kKR;
+using teuk_pt::kKD;
+using teuk_pt::kKBt;
+using teuk_pt::kKBr;
+using teuk_pt::kKC0;
+using teuk_pt::kKI0;
+using teuk_pt::kKI1;
+using teuk_pt::kKBi;
+using teuk_pt::kKCi;
+using teuk_pt::kNK;
 // Per-row values: sin^2, cos, Y, tan(Pi/2 - theta); stored [4][M].
 constexpr int kNR = 4;
 

@@ -43,31 +43,12 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The nine coefficient fields at one point (kern.cu:132-158) from the
-// column constants K[13] and the row values sin^2, cos and Y.  d_re is
-// returned without the -dbdx*ee term.
-struct Coef {
-  double bb, cc, ee, are, aim, dre, dim, vre, vim;
-};
+using teuk_pt::Coef;
 
+// The nine coefficient fields at one point (kern.cu:132-158) from the
+// column constants and the row values sin^2, cos and Y (teuk_point.h).
 __device__ __forceinline__ Coef coef_point(const double (&K)[kNK], double sn2, double cs, double Y) {
-  const double bir = K[kK2] * sn2 - K[kK1];
-  const double ib = 1.0 / bir;
-  const double Atr = K[kKA] * ib, Arr = K[kKR] * ib;
-  const double BtRe = K[kKBt] * ib, BrRe = K[kKBr] * ib;
-  const double CRe = (K[kKC0] + K[kKD] * Y) * ib;
-  const double BtIm = (K[kKI0] - K[kKI1] * cs) * ib;
-  Coef c;
-  c.bb = -0.5 * (Atr + __dsqrt_rn(Atr * Atr + 4.0 * Arr));
-  c.cc = -(K[kKD] * ib);
-  c.ee = Atr + c.bb;
-  c.are = -BtRe;
-  c.aim = -BtIm;
-  c.dre = BrRe - c.bb * BtRe;
-  c.dim = K[kKBi] * ib - c.bb * BtIm;
-  c.vre = -CRe;
-  c.vim = -(K[kKCi] * ib);
-  return c;
+  return teuk_pt::coef_point(K, sn2, cs, Y, [](double x) { return __dsqrt_rn(x); });
 }
 
 // ---------------------------------------------------------------------------
@@ -97,9 +78,8 @@ __global__ __launch_bounds__(256) void k_init_dbdx(Grid g, const double *__restr
   const int a = (int)(id % g.N);
   double d = 0.0;
   if (a >= 3 && a < g.N - 3)
-    d = (45.0 * bbp[id + 1] - 45.0 * bbp[id - 1] - 9.0 * bbp[id + 2] + 9.0 * bbp[id - 2] + bbp[id + 3] -
-         bbp[id - 3]) *
-        (kD1 * (1.0 / g.dx));
+    d = teuk_pt::d1(bbp[id + 1], bbp[id - 1], bbp[id + 2], bbp[id - 2], bbp[id + 3], bbp[id - 3],
+                    kD1 * (1.0 / g.dx));
   dbdx[id] = d;
 }
 
@@ -177,7 +157,8 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
   const bool ko_col = (a >= 4 && a < N - 4);
 
   const double dt = g.dt;
-  const double over_dx = 1.0 / g.dx, over_dth = 1.0 / g.dtheta, over_dt = 1.0 / dt;
+  const double over_dx = 1.0 / g.dx, over_dth = 1.0 / g.dtheta;
+  const double kof = teuk_pt::ko_factor(dt);
   const double wx = kD1 * over_dx;
   const double wy = 0.05 * kThird * over_dth;
   const double wyy = 0.05 * kThird * kThird * over_dth * over_dth;
@@ -329,78 +310,76 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
       const int c = lane + kHalo;
       auto dx1 = [&](int f) {
         const double *r = L + f * kLdsCols;
-        return (45.0 * r[c + 1] - 45.0 * r[c - 1] - 9.0 * r[c + 2] + 9.0 * r[c - 2] + r[c + 3] - r[c - 3]) * wx;
+        return teuk_pt::d1(r[c + 1], r[c - 1], r[c + 2], r[c - 2], r[c + 3], r[c - 3], wx);
       };
       auto ko8 = [&](int f) {
         const double *r = L + f * kLdsCols;
-        return r[c + 4] + r[c - 4] - 8.0 * (r[c + 3] + r[c - 3]) + 70.0 * r[c] - 56.0 * (r[c + 1] + r[c - 1]) +
-               28.0 * (r[c + 2] + r[c - 2]);
+        return teuk_pt::ko8(r[c], r[c + 1], r[c - 1], r[c + 2], r[c - 2], r[c + 3], r[c - 3], r[c + 4], r[c - 4]);
       };
       const double ct = late.rt[3];
-      const double qry = ((wqr[i + 6] - wqr[i]) - 9.0 * (wqr[i + 5] - wqr[i + 1]) + 45.0 * (wqr[i + 4] - wqr[i + 2])) * wy;
-      const double qryy = (2.0 * (wqr[i + 6] + wqr[i]) - 27.0 * (wqr[i + 5] + wqr[i + 1]) +
-                           270.0 * (wqr[i + 4] + wqr[i + 2]) - 490.0 * wqr[i + 3]) * wyy;
-      const double qiy = ((wqi[i + 6] - wqi[i]) - 9.0 * (wqi[i + 5] - wqi[i + 1]) + 45.0 * (wqi[i + 4] - wqi[i + 2])) * wy;
-      const double qiyy = (2.0 * (wqi[i + 6] + wqi[i]) - 27.0 * (wqi[i + 5] + wqi[i + 1]) +
-                           270.0 * (wqi[i + 4] + wqi[i + 2]) - 490.0 * wqi[i + 3]) * wyy;
-      const double llr = qryy + ct * qry;
-      const double lli = qiyy + ct * qiy;
-      const double qrx = dx1(0), qix = dx1(1), prx = dx1(2), pix = dx1(3);
-      double dqr = 0.0, dqi = 0.0, dpr = 0.0, dpi = 0.0;
+      teuk_pt::Pt x;
+      x.qr = qr0;
+      x.qi = qi0;
+      x.pr = pr0;
+      x.pi = pi0;
+      const double qry = teuk_pt::d1(wqr[i + 4], wqr[i + 2], wqr[i + 5], wqr[i + 1], wqr[i + 6], wqr[i], wy);
+      const double qryy =
+          teuk_pt::d2(wqr[i + 3], wqr[i + 4], wqr[i + 2], wqr[i + 5], wqr[i + 1], wqr[i + 6], wqr[i], wyy);
+      const double qiy = teuk_pt::d1(wqi[i + 4], wqi[i + 2], wqi[i + 5], wqi[i + 1], wqi[i + 6], wqi[i], wy);
+      const double qiyy =
+          teuk_pt::d2(wqi[i + 3], wqi[i + 4], wqi[i + 2], wqi[i + 5], wqi[i + 1], wqi[i + 6], wqi[i], wyy);
+      x.llr = teuk_pt::fma_(ct, qry, qryy);
+      x.lli = teuk_pt::fma_(ct, qiy, qiyy);
+      x.qrx = dx1(0);
+      x.qix = dx1(1);
+      x.prx = dx1(2);
+      x.pix = dx1(3);
+      x.kqr = x.kqi = x.kpr = x.kpi = 0.0;
       if (ko_col) {
-        dqr = ko8(0);
-        dqi = ko8(1);
-        dpr = ko8(2);
-        dpi = ko8(3);
+        x.kqr = ko8(0) * kof;
+        x.kqi = ko8(1) * kof;
+        x.kpr = ko8(2) * kof;
+        x.kpi = ko8(3) * kof;
       }
-      dqr = 0.1 * dqr * 0.0625 * over_dt;
-      dqi = 0.1 * dqi * 0.0625 * over_dt;
-      dpr = 0.1 * dpr * 0.0625 * over_dt;
-      dpi = 0.1 * dpi * 0.0625 * over_dt;
+      x.sr = late.sr;
+      x.si = late.si;
 
       const size_t id = c0 + a;
       const Coef cf = coef_point(K, late.rt[0], late.rt[1], late.rt[2]);
-      const double dre = cf.dre - late.dbdx * cf.ee;
-      const double sr = late.sr, si = late.si;
-
+      const double dre = teuk_pt::fma_(-late.dbdx, cf.ee, cf.dre);
       double r[4];
-      r[0] = -cf.bb * qrx + pr0 - dqr;
-      r[1] = -cf.bb * qix + pi0 - dqi;
-      r[2] = sr + cf.ee * prx - dpr + cf.cc * llr + dre * qrx - cf.dim * qix - cf.are * pr0 + cf.aim * pi0 -
-             cf.vre * qr0 + cf.vim * qi0;
-      r[3] = si + cf.ee * pix - dpi + cf.cc * lli + dre * qix + cf.dim * qrx - cf.are * pi0 - cf.aim * pr0 -
-             cf.vre * qi0 - cf.vim * qr0;
+      teuk_pt::rhs(cf, dre, x, r);
 
       const double u[4] = {qr0, qi0, pr0, pi0};
       if (INTEG == 0) {
         if (STAGE == 1) {
 #pragma unroll
-          for (int f = 0; f < 4; f++) v[f] = u[f] + dt * r[f];
+          for (int f = 0; f < 4; f++) v[f] = teuk_pt::rk3_s1(u[f], dt, r[f]);
         } else if (STAGE == 2) {
 #pragma unroll
-          for (int f = 0; f < 4; f++) v[f] = 0.25 * (3.0 * late.base[f] + u[f] + dt * r[f]);
+          for (int f = 0; f < 4; f++) v[f] = teuk_pt::rk3_s2(late.base[f], u[f], dt, r[f]);
         } else {
 #pragma unroll
-          for (int f = 0; f < 4; f++) v[f] = kThird * (late.base[f] + 2.0 * u[f] + 2.0 * dt * r[f]);
+          for (int f = 0; f < 4; f++) v[f] = teuk_pt::rk3_s3(late.base[f], u[f], dt, r[f], kThird);
         }
       } else {
         const double h2 = 0.5 * dt, h6 = dt / 6.0, h3 = dt / 3.0;
         if (STAGE == 1) {
 #pragma unroll
           for (int f = 0; f < 4; f++) {
-            v[f] = u[f] + h2 * r[f];
-            acc[f * NM + id] = u[f] + h6 * r[f];
+            v[f] = teuk_pt::axpy(h2, r[f], u[f]);
+            acc[f * NM + id] = teuk_pt::axpy(h6, r[f], u[f]);
           }
         } else if (STAGE == 2 || STAGE == 3) {
           const double h = (STAGE == 2) ? h2 : dt;
 #pragma unroll
           for (int f = 0; f < 4; f++) {
-            v[f] = late.base[f] + h * r[f];
-            acc[f * NM + id] = late.acc[f] + h3 * r[f];
+            v[f] = teuk_pt::axpy(h, r[f], late.base[f]);
+            acc[f * NM + id] = teuk_pt::axpy(h3, r[f], late.acc[f]);
           }
         } else {
 #pragma unroll
-          for (int f = 0; f < 4; f++) v[f] = late.acc[f] + h6 * r[f];
+          for (int f = 0; f < 4; f++) v[f] = teuk_pt::axpy(h6, r[f], late.acc[f]);
         }
       }
     }

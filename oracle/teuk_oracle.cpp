@@ -6,8 +6,10 @@
  * follows.  The product library never links this file; tests, the smoke
  * check and bench.py's cpu_baseline leg load it as the checker.
  *
- * Parity build:  g++ -O2 -ffp-contract=off  (no FMA contraction, so sums are
- *                evaluated in the order written, as in the reference source)
+ * Parity build:  g++ -O2 -ffp-contract=off  (no implicit FMA contraction: the
+ *                point arithmetic of teuk_point.h, shared with the device,
+ *                places its fused multiply-adds explicitly, everything else
+ *                is evaluated in the order written)
  * Timing build:  g++ -O3 -fopenmp           (cpu_baseline leg of bench.py)
  *
  * Pinning (how each function is checked, in tests/):
@@ -17,9 +19,8 @@
  *   orc_fact/orc_swsh/orc_project
  *                 bitwise against the reference proj.c compiled into
  *                 oracle/_ref, plus closed forms of -2Y_lm and orthonormality.
- *   orc_coeffs    against an independent symbolic derivation of the Teukolsky
- *                 operator in the compactified coordinates (tests/golden/
- *                 coeff_derivation.py), plus the Schwarzschild/Omega=1 limit.
+ *   orc_coeffs    parity unpinned: a restatement of kern.cu:132-158 in
+ *                 factored form; no reference output exists to compare with.
  *   orc_rhs       stencil exactness on polynomials, KO annihilation of degree
  *                 <= 7, assembly against a direct evaluation on smooth fields.
  *   orc_bc_*      exact extrapolation formulas and ordering invariants.
@@ -29,6 +30,7 @@
  *                 canonical form and physical checks (see teuk_source.inc).
  */
 #include "teuk_oracle.h"
+#include "teuk_point.h"
 
 #include <cmath>
 #include <cstdlib>
@@ -134,7 +136,7 @@ void orc_coeffs(const orc_params *p, const double *theta, const double *r_c,
    * as (column constant [+ column constant x row value]) / bir, with
    * bir = K2 sin^2 - K1; the algebra is that of kern.cu:132-145 term by
    * term, only regrouped. */
-  enum { K1, K2, KA, KR, KD, KBt, KBr, KC0, KI0, KI1, KBi, KCi, NK };
+  enum { K1, K2, KA, KR, KD, KBt, KBr, KC0, KI0, KI1, KBi, KCi, NK };   /* teuk_point.h order */
   std::vector<double> K((size_t)NK * N);
   for (int j = 0; j < N; j++) {
     const double x = p->xmin + (double)j * p->dx;
@@ -187,24 +189,19 @@ void orc_coeffs(const orc_params *p, const double *theta, const double *r_c,
     const double sn2 = sn * sn;
     const double Y = 4.0 * (cot * cot) - 4.0 * em * cot * csc + (em * em) * (csc * csc);
     for (int j = 0; j < N; j++) {
-      auto k_ = [&](int which) { return K[(size_t)which * N + j]; };
-      const double bir = k_(K2) * sn2 - k_(K1);
-      const double ib = 1.0 / bir;
-      const double Atr = k_(KA) * ib, Arr = k_(KR) * ib;
-      const double BtRe = k_(KBt) * ib, BrRe = k_(KBr) * ib;
-      const double CRe = (k_(KC0) + k_(KD) * Y) * ib;
-      const double BtIm = (k_(KI0) - k_(KI1) * cs) * ib;
-      const double bb = -0.5 * (Atr + std::sqrt(Atr * Atr + 4.0 * Arr));
+      double Kj[NK];
+      for (int i = 0; i < NK; i++) Kj[i] = K[(size_t)i * N + j];
+      const teuk_pt::Coef cf = teuk_pt::coef_point(Kj, sn2, cs, Y, [](double v) { return std::sqrt(v); });
       const size_t id = (size_t)k * N + j;
-      coef[0 * NM + id] = bb;
-      coef[1 * NM + id] = -(k_(KD) * ib);          /* A^yy = -Delta/bir */
-      coef[2 * NM + id] = Atr + bb;
-      coef[3 * NM + id] = -BtRe;
-      coef[4 * NM + id] = -BtIm;
-      coef[5 * NM + id] = BrRe - bb * BtRe;
-      coef[6 * NM + id] = k_(KBi) * ib - bb * BtIm;
-      coef[7 * NM + id] = -CRe;
-      coef[8 * NM + id] = -(k_(KCi) * ib);
+      coef[0 * NM + id] = cf.bb;
+      coef[1 * NM + id] = cf.cc;            /* A^yy = -Delta/bir */
+      coef[2 * NM + id] = cf.ee;
+      coef[3 * NM + id] = cf.are;
+      coef[4 * NM + id] = cf.aim;
+      coef[5 * NM + id] = cf.dre;
+      coef[6 * NM + id] = cf.dim;
+      coef[7 * NM + id] = cf.vre;
+      coef[8 * NM + id] = cf.vim;
     }
   }
 }
@@ -223,27 +220,26 @@ void orc_rhs(const orc_params *p, const double *coef, const double *theta,
   const double *vre = coef + 7 * NM, *vim = coef + 8 * NM;
   const double over_dx = 1.0 / p->dx;
   const double over_dth = 1.0 / p->dtheta;
-  const double over_dt = 1.0 / p->dt;
+  const double kof = teuk_pt::ko_factor(p->dt);
   const double wy = 0.05 * kThird * over_dth;
   const double wyy = 0.05 * kThird * kThird * over_dth * over_dth;
   const double wx = kD1 * over_dx;
 
+  /* Point arithmetic (stencils with explicit fused multiply-adds, RHS
+   * assembly) from teuk_point.h, the same code the device compiles. */
   auto dx1 = [&](const double *f, size_t c) {
-    return (45.0 * f[c + 1] - 45.0 * f[c - 1] - 9.0 * f[c + 2] + 9.0 * f[c - 2]
-            + f[c + 3] - f[c - 3]) * wx;
+    return teuk_pt::d1(f[c + 1], f[c - 1], f[c + 2], f[c - 2], f[c + 3], f[c - 3], wx);
   };
   auto ko8 = [&](const double *f, size_t c) {
-    return f[c + 4] + f[c - 4] - 8.0 * (f[c + 3] + f[c - 3]) + 70.0 * f[c]
-           - 56.0 * (f[c + 1] + f[c - 1]) + 28.0 * (f[c + 2] + f[c - 2]);
+    return teuk_pt::ko8(f[c], f[c + 1], f[c - 1], f[c + 2], f[c - 2], f[c + 3], f[c - 3], f[c + 4], f[c - 4]);
   };
   const size_t sN = (size_t)N;
   auto dy1 = [&](const double *f, size_t c) {
-    return ((f[c + 3 * sN] - f[c - 3 * sN]) - 9.0 * (f[c + 2 * sN] - f[c - 2 * sN])
-            + 45.0 * (f[c + sN] - f[c - sN])) * wy;
+    return teuk_pt::d1(f[c + sN], f[c - sN], f[c + 2 * sN], f[c - 2 * sN], f[c + 3 * sN], f[c - 3 * sN], wy);
   };
   auto dy2 = [&](const double *f, size_t c) {
-    return (2.0 * (f[c + 3 * sN] + f[c - 3 * sN]) - 27.0 * (f[c + 2 * sN] + f[c - 2 * sN])
-            + 270.0 * (f[c + sN] + f[c - sN]) - 490.0 * f[c]) * wyy;
+    return teuk_pt::d2(f[c], f[c + sN], f[c - sN], f[c + 2 * sN], f[c - 2 * sN], f[c + 3 * sN], f[c - 3 * sN],
+                       wyy);
   };
 
   const int nth = p->nthreads > 1 ? p->nthreads : 1;
@@ -253,27 +249,29 @@ void orc_rhs(const orc_params *p, const double *coef, const double *theta,
     for (int a = 3; a < N - 3; a++) {
       const size_t c = (size_t)b * N + a;
       const double dbdx = dx1(bb, c);
-      const double dre = dre_c[c] - dbdx * ee[c];
-      const double llr = dy2(qr, c) + ctan * dy1(qr, c);
-      const double lli = dy2(qi, c) + ctan * dy1(qi, c);
-      const double qrx = dx1(qr, c), qix = dx1(qi, c);
-      const double prx = dx1(pr, c), pix = dx1(pi, c);
-      double dqr = 0.0, dqi = 0.0, dpr = 0.0, dpi = 0.0;
+      teuk_pt::Coef cf;
+      cf.bb = bb[c]; cf.cc = cc[c]; cf.ee = ee[c]; cf.are = are[c]; cf.aim = aim[c];
+      cf.dre = dre_c[c]; cf.dim = dim[c]; cf.vre = vre[c]; cf.vim = vim[c];
+      const double dre = teuk_pt::fma_(-dbdx, cf.ee, cf.dre);
+      teuk_pt::Pt x;
+      x.qr = qr[c]; x.qi = qi[c]; x.pr = pr[c]; x.pi = pi[c];
+      x.llr = teuk_pt::fma_(ctan, dy1(qr, c), dy2(qr, c));
+      x.lli = teuk_pt::fma_(ctan, dy1(qi, c), dy2(qi, c));
+      x.qrx = dx1(qr, c); x.qix = dx1(qi, c);
+      x.prx = dx1(pr, c); x.pix = dx1(pi, c);
+      x.kqr = x.kqi = x.kpr = x.kpi = 0.0;
       if (a >= 4 && a < N - 4) {
-        dqr = ko8(qr, c); dqi = ko8(qi, c); dpr = ko8(pr, c); dpi = ko8(pi, c);
+        x.kqr = ko8(qr, c) * kof; x.kqi = ko8(qi, c) * kof;
+        x.kpr = ko8(pr, c) * kof; x.kpi = ko8(pi, c) * kof;
       }
-      dqr = 0.1 * dqr * 0.0625 * over_dt;
-      dqi = 0.1 * dqi * 0.0625 * over_dt;
-      dpr = 0.1 * dpr * 0.0625 * over_dt;
-      dpi = 0.1 * dpi * 0.0625 * over_dt;
-      const double sr = src ? src[c] : 0.0;
-      const double si = src ? src[NM + c] : 0.0;
-      rhs[c] = -bb[c] * qrx + pr[c] - dqr;
-      rhs[NM + c] = -bb[c] * qix + pi[c] - dqi;
-      rhs[2 * NM + c] = sr + ee[c] * prx - dpr + cc[c] * llr + dre * qrx - dim[c] * qix
-                        - are[c] * pr[c] + aim[c] * pi[c] - vre[c] * qr[c] + vim[c] * qi[c];
-      rhs[3 * NM + c] = si + ee[c] * pix - dpi + cc[c] * lli + dre * qix + dim[c] * qrx
-                        - are[c] * pi[c] - aim[c] * pr[c] - vre[c] * qi[c] - vim[c] * qr[c];
+      x.sr = src ? src[c] : 0.0;
+      x.si = src ? src[NM + c] : 0.0;
+      double r[4];
+      teuk_pt::rhs(cf, dre, x, r);
+      rhs[c] = r[0];
+      rhs[NM + c] = r[1];
+      rhs[2 * NM + c] = r[2];
+      rhs[3 * NM + c] = r[3];
     }
   }
 }
@@ -345,27 +343,27 @@ void orc_step(const orc_params *p, const double *coef, const double *theta,
   };
   if (p->integrator == 0) {
     orc_rhs(p, coef, theta, u, src, rhs);
-    interior([&](size_t c) { uc[c] = u[c] + dt * rhs[c]; });
+    interior([&](size_t c) { uc[c] = teuk_pt::rk3_s1(u[c], dt, rhs[c]); });
     orc_bc_theta(p, uc); orc_bc_rho(p, uc);
     orc_rhs(p, coef, theta, uc, src, rhs);
-    interior([&](size_t c) { uc[c] = 0.25 * (3.0 * u[c] + uc[c] + dt * rhs[c]); });
+    interior([&](size_t c) { uc[c] = teuk_pt::rk3_s2(u[c], uc[c], dt, rhs[c]); });
     orc_bc_theta(p, uc); orc_bc_rho(p, uc);
     orc_rhs(p, coef, theta, uc, src, rhs);
-    interior([&](size_t c) { u[c] = kThird * (u[c] + 2.0 * uc[c] + 2.0 * dt * rhs[c]); });
+    interior([&](size_t c) { u[c] = teuk_pt::rk3_s3(u[c], uc[c], dt, rhs[c], kThird); });
     orc_bc_theta(p, u); orc_bc_rho(p, u);
   } else {
     const double h2 = 0.5 * dt, h6 = dt / 6.0, h3 = dt / 3.0;
     orc_rhs(p, coef, theta, u, src, rhs);
-    interior([&](size_t c) { uc[c] = u[c] + h2 * rhs[c]; acc[c] = u[c] + h6 * rhs[c]; });
+    interior([&](size_t c) { uc[c] = teuk_pt::axpy(h2, rhs[c], u[c]); acc[c] = teuk_pt::axpy(h6, rhs[c], u[c]); });
     orc_bc_theta(p, uc); orc_bc_rho(p, uc);
     orc_rhs(p, coef, theta, uc, src, rhs);
-    interior([&](size_t c) { uc[c] = u[c] + h2 * rhs[c]; acc[c] = acc[c] + h3 * rhs[c]; });
+    interior([&](size_t c) { uc[c] = teuk_pt::axpy(h2, rhs[c], u[c]); acc[c] = teuk_pt::axpy(h3, rhs[c], acc[c]); });
     orc_bc_theta(p, uc); orc_bc_rho(p, uc);
     orc_rhs(p, coef, theta, uc, src, rhs);
-    interior([&](size_t c) { uc[c] = u[c] + dt * rhs[c]; acc[c] = acc[c] + h3 * rhs[c]; });
+    interior([&](size_t c) { uc[c] = teuk_pt::axpy(dt, rhs[c], u[c]); acc[c] = teuk_pt::axpy(h3, rhs[c], acc[c]); });
     orc_bc_theta(p, uc); orc_bc_rho(p, uc);
     orc_rhs(p, coef, theta, uc, src, rhs);
-    interior([&](size_t c) { u[c] = acc[c] + h6 * rhs[c]; });
+    interior([&](size_t c) { u[c] = teuk_pt::axpy(h6, rhs[c], acc[c]); });
     orc_bc_theta(p, u); orc_bc_rho(p, u);
   }
 }

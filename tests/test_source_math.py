@@ -50,6 +50,11 @@ def test_oracle_and_library_share_one_jet_header():
                        os.path.join(ROOT, "bh_ringdown_amd", "csrc", "teuk_t4.h"), shallow=False)
 
 
+def test_oracle_and_library_share_one_point_arithmetic_header():
+    assert filecmp.cmp(os.path.join(ROOT, "oracle", "teuk_point.h"),
+                       os.path.join(ROOT, "bh_ringdown_amd", "csrc", "teuk_point.h"), shallow=False)
+
+
 def test_portable_exp():
     xs = np.concatenate([-np.logspace(-12, 2.8, 4000), [0.0, -1e-300, -700.0]])
     worst = 0.0

@@ -102,20 +102,23 @@ def prepare(req, integrator=1):
 class Server:
     """Continuous batching over a libteuk batch with an HBM budget.
 
-    Admission is first come, first served, limited by the HBM budget and by
-    an occupancy target: once the resident solves together update
-    `target_points` grid points per step (enough wavefronts to keep every
-    compute unit streaming), admitting more would not raise throughput and
-    would only stretch every resident solve's latency, so later requests
-    wait for a resident one to finish."""
+    Admission is limited by the HBM budget and by an occupancy target: once
+    the resident solves together update `target_points` grid points per step
+    (enough wavefronts to keep every compute unit streaming), admitting more
+    would not raise throughput and would only stretch every resident solve's
+    latency, so later requests wait for a resident one to finish.  Waiting
+    requests are admitted shortest first (grid points x steps) when
+    `shortest_first`, else in arrival order."""
 
-    def __init__(self, hbm_budget, chunk=64, integrator=1, tiling=None, max_resident=64, target_points=3 << 20):
+    def __init__(self, hbm_budget, chunk=64, integrator=1, tiling=None, max_resident=64, target_points=2 << 20,
+                 shortest_first=True):
         self.hbm_budget = int(hbm_budget)
         self.chunk = chunk
         self.integrator = integrator
         self.tiling = tiling or {}
         self.max_resident = max_resident
         self.target_points = int(target_points)
+        self.shortest_first = shortest_first
 
     def run(self, requests, clock=time.perf_counter, on_records=None):
         """Serve `requests` (sorted by arrival); returns {id: Result}."""
@@ -138,6 +141,8 @@ class Server:
                 # 2. meanwhile: arrivals and admission by footprint
                 while pending and pending[0].arrival <= now():
                     queue.append(pending.pop(0))
+                if self.shortest_first:
+                    queue.sort(key=lambda r: (r.N * r.M * r.t_final / ((r.xmax - r.xmin) / r.N), r.arrival))
                 while queue and len(live) < self.max_resident:
                     resident = sum(v[1].N * v[1].M for v in live.values())
                     if live and resident >= self.target_points:

@@ -110,7 +110,7 @@ class Server:
     requests are admitted shortest first (grid points x steps) when
     `shortest_first`, else in arrival order."""
 
-    def __init__(self, hbm_budget, chunk=64, integrator=1, tiling=None, max_resident=64, target_points=2 << 20,
+    def __init__(self, hbm_budget, chunk=128, integrator=1, tiling=None, max_resident=64, target_points=1 << 20,
                  shortest_first=True):
         self.hbm_budget = int(hbm_budget)
         self.chunk = chunk
@@ -119,6 +119,7 @@ class Server:
         self.max_resident = max_resident
         self.target_points = int(target_points)
         self.shortest_first = shortest_first
+        self.stats = {}
 
     def run(self, requests, clock=time.perf_counter, on_records=None):
         """Serve `requests` (sorted by arrival); returns {id: Result}."""
@@ -128,6 +129,9 @@ class Server:
         live = {}             # id -> (req, params, solver, records read)
         t0 = clock()
         now = lambda: clock() - t0
+        # host time spent preparing requests, enqueuing chunks, waiting for
+        # the GPU and streaming records back
+        st = self.stats = {"prepare_s": 0.0, "enqueue_s": 0.0, "sync_s": 0.0, "stream_s": 0.0, "chunks": 0}
         batch = teuk.Batch(integrator=self.integrator, hbm_budget=self.hbm_budget)
         if self.tiling:
             batch.set_tiling(chunk=self.chunk, **self.tiling)
@@ -137,7 +141,10 @@ class Server:
             while pending or queue or live:
                 # 1. enqueue a chunk for everything resident (asynchronous)
                 if live:
+                    ts = clock()
                     batch.step(self.chunk)
+                    st["enqueue_s"] += clock() - ts
+                    st["chunks"] += 1
                 # 2. meanwhile: arrivals and admission by footprint
                 while pending and pending[0].arrival <= now():
                     queue.append(pending.pop(0))
@@ -153,7 +160,9 @@ class Server:
                         raise ValueError(f"request {req.id} alone exceeds the HBM budget")
                     if batch.info()["bytes"] + teuk.footprint(p) > self.hbm_budget:
                         break
+                    ts = clock()
                     p, s = prepare(req, self.integrator)
+                    st["prepare_s"] += clock() - ts
                     if not batch.add(s):
                         s.close()
                         break
@@ -165,7 +174,10 @@ class Server:
                         time.sleep(max(0.0, min(0.05, pending[0].arrival - now())))
                     continue
                 # 3. chunk done: stream records, retire finished solves
+                ts = clock()
                 batch.sync()
+                st["sync_s"] += clock() - ts
+                ts = clock()
                 for rid in list(live):
                     req, p, s, nread = live[rid]
                     out = s.output(first=nread)
@@ -187,6 +199,7 @@ class Server:
                         batch.remove(s)
                         s.close()
                         del live[rid]
+                st["stream_s"] += clock() - ts
         finally:
             for req, p, s, _ in live.values():
                 s.close()

@@ -37,6 +37,16 @@
 namespace teuk {
 namespace {
 
+// Global accesses by a 32-bit byte offset from a uniform base: the address is
+// an SGPR pair plus one VGPR (no 64-bit address arithmetic per access).  A
+// solve's planes stay below 4 GiB (teuk_create checks N*M).
+__device__ __forceinline__ double ld(const double *base, unsigned off) {
+  return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + off);
+}
+__device__ __forceinline__ void st(double *base, unsigned off, double v) {
+  *reinterpret_cast<double *>(reinterpret_cast<char *>(base) + off) = v;
+}
+
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -130,6 +140,7 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
   __shared__ double lds[4][4][kLdsCols];
   const int N = g.N, M = g.M;
   const size_t NM = (size_t)N * M;
+  const unsigned pb = 8u * (unsigned)NM;   // bytes of one plane
   // The wavefront index is uniform: make that visible to the compiler so the
   // row index b is a scalar and the per-row tables are scalar loads.
   const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y);
@@ -223,9 +234,9 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
   // Store one finished row (no conditions: the compiler's memory wait counts
   // stay exact across rows).
   auto store_row = [&](int b, const double (&v)[4]) {
-    const size_t c = (size_t)b * N + a;
+    const unsigned c = 8u * (unsigned)(b * N + a);
 #pragma unroll
-    for (int f = 0; f < 4; f++) out[f * NM + c] = v[f];
+    for (int f = 0; f < 4; f++) st(out, f * pb + c, v[f]);
   };
 
   // Global inputs of a row.  The stencil inputs (p and the halo columns) are
@@ -254,21 +265,21 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
   constexpr bool kAccIn = INTEG == 1 && STAGE >= 2;
   auto load_row = [&](int b) {
     RowIn x;
-    const size_t c0 = (size_t)b * N;
-    x.pr = in[2 * NM + c0 + a];
-    x.pi = in[3 * NM + c0 + a];
+    const unsigned c0 = 8u * (unsigned)(b * N);
+    x.pr = ld(in, 2 * pb + c0 + 8u * a);
+    x.pi = ld(in, 3 * pb + c0 + 8u * a);
 #pragma unroll
-    for (int f = 0; f < 4; f++) x.halo[f] = in[f * NM + c0 + hsafe];
+    for (int f = 0; f < 4; f++) x.halo[f] = ld(in, f * pb + c0 + 8u * hsafe);
     return x;
   };
   auto load_late = [&](int b) {
     RowLate x;
-    const size_t c = (size_t)b * N + a;
+    const unsigned c = 8u * (unsigned)(b * N + a);
 #pragma unroll
-    for (int f = 0; f < 4; f++) x.base[f] = kBase ? base[f * NM + c] : 0.0;
+    for (int f = 0; f < 4; f++) x.base[f] = kBase ? ld(base, f * pb + c) : 0.0;
 #pragma unroll
-    for (int f = 0; f < 4; f++) x.acc[f] = kAccIn ? acc[f * NM + c] : 0.0;
-    x.dbdx = dbdx[c];
+    for (int f = 0; f < 4; f++) x.acc[f] = kAccIn ? ld(acc, f * pb + c) : 0.0;
+    x.dbdx = ld(dbdx, c);
 #pragma unroll
     for (int j = 0; j < kNR; j++) x.rt[j] = rtab[j * M + b];
     const int sb = b - it0, sa = a - ip0;
@@ -289,7 +300,6 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
     for (int i = 0; i < kRows; i++) {
     const int b = b0 + i;
     if (kEdge && (b < 3 || b >= M - 3)) continue;  // uniform over the wavefront
-    const size_t c0 = (size_t)b * N;
     const RowLate late = load_late(b);
     RowIn nxt = cur;
     if (i + 1 < kRows) nxt = load_row(kEdge && b + 1 >= M - 3 ? b : b + 1);
@@ -344,7 +354,7 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
       x.sr = late.sr;
       x.si = late.si;
 
-      const size_t id = c0 + a;
+      const unsigned id = (unsigned)(b * N + a);
       const Coef cf = coef_point(K, late.rt[0], late.rt[1], late.rt[2]);
       const double dre = teuk_pt::fma_(-late.dbdx, cf.ee, cf.dre);
       double r[4];
@@ -368,14 +378,14 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
 #pragma unroll
           for (int f = 0; f < 4; f++) {
             v[f] = teuk_pt::axpy(h2, r[f], u[f]);
-            acc[f * NM + id] = teuk_pt::axpy(h6, r[f], u[f]);
+            st(acc, f * pb + 8u * id, teuk_pt::axpy(h6, r[f], u[f]));
           }
         } else if (STAGE == 2 || STAGE == 3) {
           const double h = (STAGE == 2) ? h2 : dt;
 #pragma unroll
           for (int f = 0; f < 4; f++) {
             v[f] = teuk_pt::axpy(h, r[f], late.base[f]);
-            acc[f * NM + id] = teuk_pt::axpy(h3, r[f], late.acc[f]);
+            st(acc, f * pb + 8u * id, teuk_pt::axpy(h3, r[f], late.acc[f]));
           }
         } else {
 #pragma unroll

@@ -113,3 +113,17 @@ def test_trajectory_consumption_pattern():
     with pytest.raises(teuk.TeukError) as e:
         teuk.trajectory_schedule(rows, 30, dt)      # needs 2 + 60 rows
     assert e.value.code == teuk.TEUK_ERANGE
+
+
+def test_driver_rejects_unknown_parameter(tmp_path):
+    """teuk_run reads the reference's parameter names only (parm.h, main.c)
+    and stops before any device work on an unknown key."""
+    import subprocess
+    driver = os.path.join(ROOT, "bh_ringdown_amd", "bin", "teuk_run")
+    if not os.path.exists(driver):
+        pytest.skip("driver not built")
+    params = tmp_path / "p.txt"
+    params.write_text("N 256\nNtheta 16\n")
+    res = subprocess.run([driver, "-p", str(params)], capture_output=True, text=True, timeout=60)
+    assert res.returncode == 2
+    assert "unknown parameter 'Ntheta'" in res.stderr

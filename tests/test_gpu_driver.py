@@ -1,0 +1,71 @@
+"""The C++ driver end to end (bin/teuk_run: parameter file, Trajectories/
+<name>.traj, hm0<m>_<name>.dat and the full-precision sidecar), against the
+oracle run on the same trajectory: every written row must be the oracle's
+row, bit for bit in the sidecar and character for character in the
+reference-format file (body.cu:204-212: "%lf %g x14" for k % Fr == 0 and
+k dt >= Xmax + 200, time column timer - Xmax)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from bh_ringdown_amd import teuk
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "bh_ringdown_amd", "bin", "teuk_run")
+
+
+def oracle_rows(p, rows, nt, m):
+    """Written rows (t - Xmax, h[14]) of the oracle for mode m."""
+    q = teuk.default_params(p["N"], p["M"], p["Xmin"], p["Xmax"], p["aa"], m, dt_over_dx=0.5)
+    q.dt = p["dt"]
+    _, r_c, th = teuk.grid(q)
+    sched = teuk.trajectory_schedule(rows, nt, q.dt)
+    op = O.make_params(q.N, q.M, q.xmin, q.xmax, q.a, m, dt=q.dt)
+    ost = [O.SrcStep(*[getattr(x, f) for f, _ in teuk.SourceStep._fields_]) for x in sched]
+    u = np.zeros((4, q.M, q.N))
+    rc, psi, h, dedt = O.run(op, r_c, th, u, nt, src_steps=ost)
+    assert rc == 0
+    out, t = [], 0.0
+    for k in range(nt):
+        if k % p["Fr"] == 0 and k * q.dt >= q.xmax + 200.0:
+            out.append((t - q.xmax, h[k]))
+        t += q.dt
+    return out
+
+
+def test_driver_writes_reference_format(tmp_path):
+    assert os.path.exists(DRIVER), "driver not built (bh_ringdown_amd.build)"
+    N, M, xmin, xmax, a = 256, 16, -40.0, 60.0, 0.6
+    dt = 0.5 * (xmax - xmin) / N
+    T = 300.0
+    p = {"N": N, "M": M, "Xmin": xmin, "Xmax": xmax, "aa": a, "dt": dt, "T": T, "Fr": 10}
+    nt = int(T / dt)
+    rows = teuk.plunge_trajectory(a, 1e-2, 0.3, dt / 2, (2 * nt + 6) * dt / 2, seed=5, r0_over_isco=1.3)
+    (tmp_path / "Trajectories").mkdir()
+    teuk.write_traj(tmp_path / "Trajectories" / "plunge.traj", rows)
+    rows_back = np.loadtxt(tmp_path / "Trajectories" / "plunge.traj")
+    params = tmp_path / "params.txt"
+    params.write_text("\n".join([f"N {N}", f"M {M}", f"Xmin {xmin!r}", f"Xmax {xmax!r}", f"aa {a!r}",
+                                 f"dt {dt!r}", f"T {T!r}", "Fr 10", "PTSX 128", "PTSY 128", "mmin 2", "mmax 3",
+                                 "integrator 0", "j_extract -1"]) + "\n")
+    print("driver: teuk_run on 256x16, m = 2, 3", flush=True)
+    res = subprocess.run([DRIVER, "-p", str(params), "-t", str(tmp_path / "Trajectories"), "-o", str(tmp_path),
+                          "plunge"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    assert "WELCOME TO TEUKOLSKY CODE!" in res.stdout
+    for m, name in ((2, "hm02_plunge"), (3, "hm03_plunge")):
+        want = oracle_rows(p, rows_back, nt, m)
+        assert len(want) > 5
+        full = np.loadtxt(tmp_path / f"{name}.full.dat")
+        assert full.shape == (len(want), 15)
+        for (t, h), row in zip(want, full):
+            assert row[0] == t and np.array_equal(row[1:], h)
+        lines = (tmp_path / f"{name}.dat").read_text().splitlines()
+        assert len(lines) == len(want)
+        for (t, h), line in zip(want, lines):
+            assert line == "%f" % t + "".join(" %g" % v for v in h)

@@ -1,0 +1,74 @@
+"""Self-convergence of psi4 over three resolutions on one GPU.
+
+    python -m bh_ringdown_amd.convergence [--a 0.99] [--t-final 2000] [--base 16384x32]
+
+Runs the same seeded plunge (mass ratio 1e-3, inclination 30 degrees, from
+1.02 r_isco) on rho in [-600, 1600] at (N, M), (2N, 2M) and (4N, 4M) with
+dt = 0.5 drho and classic RK4, extracts psi4 at the same rho (a column that
+lies on all three grids) and the theta rows the three grids share, and
+reports the convergence ratio |psi_1 - psi_2| / |psi_2 - psi_4| over time
+windows (2^p for order p).
+"""
+import argparse
+import json
+import math
+
+import numpy as np
+
+from . import teuk, workloads
+
+
+def run(N, M, a, t_final, j_extract, seed=0, xmin=-600.0, xmax=1600.0):
+    p = teuk.default_params(N, M, xmin, xmax, a, 2, dt_over_dx=0.5, T=t_final)
+    p.integrator = 1
+    p.Fr = 1
+    p.j_extract = j_extract
+    nt = p.nsteps()
+    rows = workloads.plunge_rows(p, nt, eta=1e-3, iota_deg=30.0, seed=seed)
+    with teuk.Solver(p) as s:
+        s.set_trajectory(rows, nt)
+        s.step(nt)
+        s.sync()
+        out = s.output()
+        psi = s.psi()
+    return p, out, psi
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--a", type=float, default=0.99)
+    ap.add_argument("--t-final", type=float, default=2000.0)
+    ap.add_argument("--base", default="16384x32")
+    ap.add_argument("--rho", type=float, default=1000.0)
+    ap.add_argument("--windows", type=int, default=8)
+    args = ap.parse_args(argv)
+    N, M = map(int, args.base.split("x"))
+    xmin, xmax = -600.0, 1600.0
+    j0 = int(round((args.rho - xmin) / ((xmax - xmin) / N)))
+    res = []
+    for k in range(3):
+        f = 2 ** k
+        p, out, psi = run(N * f, M * f, args.a, args.t_final, j0 * f)
+        # coarse time levels (every f-th step) and shared theta rows (every f-th row)
+        psi_c = psi[::f, :, ::f]
+        res.append((p, out["t"][::f], psi_c[:, 0] + 1j * psi_c[:, 1]))
+        print(json.dumps({"N": p.N, "M": p.M, "steps": p.nsteps(), "rho": xmin + j0 * f * p.dx}), flush=True)
+    n = min(len(r[1]) for r in res)
+    t = res[0][1][:n]
+    d12 = np.abs(res[0][2][:n] - res[1][2][:n])
+    d24 = np.abs(res[1][2][:n] - res[2][2][:n])
+    amp = np.abs(res[2][2][:n])
+    edges = np.linspace(0, n, args.windows + 1).astype(int)
+    report = []
+    for lo, hi in zip(edges[:-1], edges[1:]):
+        e12 = float(np.sqrt(np.mean(d12[lo:hi] ** 2)))
+        e24 = float(np.sqrt(np.mean(d24[lo:hi] ** 2)))
+        report.append({"t": [float(t[lo]), float(t[hi - 1])], "rms_psi": float(np.sqrt(np.mean(amp[lo:hi] ** 2))),
+                       "diff_12": e12, "diff_24": e24,
+                       "order": (math.log2(e12 / e24) if e24 > 0 and e12 > 0 else None)})
+    print(json.dumps({"a": args.a, "t_final": args.t_final, "base": args.base, "windows": report}), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

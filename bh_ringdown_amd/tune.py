@@ -27,18 +27,26 @@ from . import teuk
 
 CACHE = os.environ.get("TEUK_TUNE_CACHE") or os.path.join(os.path.expanduser("~"), ".cache", "bh_ringdown_amd",
                                                           "tuning.json")
+# Results of a tuning run on an MI355X, used where the local cache has no entry.
+DEFAULTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_defaults.json")
 
 
-def _load():
+def _read(path):
     try:
-        with open(CACHE) as f:
+        with open(path) as f:
             return json.load(f)
     except (OSError, ValueError):
         return {}
 
 
+def _load():
+    data = _read(DEFAULTS)
+    data.update(_read(CACHE))
+    return data
+
+
 def _store(key, value):
-    data = _load()
+    data = _read(CACHE)
     data[key] = value
     os.makedirs(os.path.dirname(CACHE), exist_ok=True)
     tmp = CACHE + ".tmp"

@@ -46,6 +46,8 @@ def main(argv=None):
     ap.add_argument("--base", default="16384x32")
     ap.add_argument("--rho", type=float, default=1000.0)
     ap.add_argument("--windows", type=int, default=8)
+    ap.add_argument("--refine", choices=["both", "rho", "theta"], default="both",
+                    help="which directions are refined (dt follows drho)")
     ap.add_argument("--no-source", action="store_true",
                     help="Gaussian initial data (width 8 M at rho = 0) instead of the plunge source")
     args = ap.parse_args(argv)
@@ -55,10 +57,14 @@ def main(argv=None):
     res = []
     for k in range(3):
         f = 2 ** k
-        p, out, psi = run(N * f, M * f, args.a, args.t_final, j0 * f, source=not args.no_source)
-        # coarse time levels (every f-th step) and shared theta rows (every f-th row)
-        psi_c = psi[::f, :, ::f]
-        res.append((p, out["t"][::f], psi_c[:, 0] + 1j * psi_c[:, 1]))
+        fr = f if args.refine in ("both", "rho") else 1
+        ft = f if args.refine in ("both", "theta") else 1
+        p, out, psi = run(N * fr, M * ft, args.a, args.t_final, j0 * fr, source=not args.no_source)
+        # Record k holds psi after step k, at time (k + 1) dt: the coarse
+        # record k matches the refined record fr (k + 1) - 1.  Shared theta
+        # rows: every ft-th row.
+        psi_c = psi[fr - 1::fr, :, ::ft]
+        res.append((p, out["t"][fr - 1::fr] + p.dt, psi_c[:, 0] + 1j * psi_c[:, 1]))
         print(json.dumps({"N": p.N, "M": p.M, "steps": p.nsteps(), "rho": xmin + j0 * f * p.dx}), flush=True)
     n = min(len(r[1]) for r in res)
     t = res[0][1][:n]
@@ -73,7 +79,8 @@ def main(argv=None):
         report.append({"t": [float(t[lo]), float(t[hi - 1])], "rms_psi": float(np.sqrt(np.mean(amp[lo:hi] ** 2))),
                        "diff_12": e12, "diff_24": e24,
                        "order": (math.log2(e12 / e24) if e24 > 0 and e12 > 0 else None)})
-    print(json.dumps({"a": args.a, "t_final": args.t_final, "base": args.base, "source": not args.no_source,
+    print(json.dumps({"a": args.a, "t_final": args.t_final, "base": args.base, "refine": args.refine,
+                      "source": not args.no_source,
                       "rho": args.rho, "windows": report}), flush=True)
     return 0
 

@@ -35,6 +35,15 @@ STAGES = {"rk3": 3, "rk4": 4}
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_stage_rk4_16384x64_plunge_pmc.txt")
 
 
+# Timed single solves: name -> (maker in bh_ringdown_amd.workloads, its grid size).
+WORKLOADS = {
+    "single_16384x64_plunge": ("single_16384x64", None),
+    "high_spin_16384x32_plunge": ("high_spin", (16384, 32)),
+    "high_spin_32768x64_plunge": ("high_spin", (32768, 64)),
+    "high_spin_65536x128_plunge": ("high_spin", (65536, 128)),
+}
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -160,6 +169,9 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--integrator", choices=["rk4", "rk3"], default="rk4")
+    ap.add_argument("--workload", choices=list(WORKLOADS), default="single_16384x64_plunge",
+                    help="the single solve timed (the default is the headline configuration; the a=0.99 "
+                         "ladder grids are larger single solves of the same path)")
     ap.add_argument("--wpb", type=int, default=0, help="wavefronts per block (0 = default)")
     ap.add_argument("--wpe", type=int, default=0, help="occupancy target, waves per SIMD (0 = default)")
     ap.add_argument("--no-batch", action="store_true")
@@ -186,14 +198,15 @@ def main():
         log("bench: copy calibration, 4 x (2 GiB read + 2 GiB written)")
         del x, y
     integ = 1 if args.integrator == "rk4" else 0
-    p = workloads.single_16384x64(integrator=integ)
+    maker, size = WORKLOADS[args.workload]
+    p = getattr(workloads, maker)(*(size or ()), integrator=integ)
     need = args.warmup + args.steps + args.timing_steps
     p.T = max(p.T, need * p.dt * (1.0 + 1e-12))
     nt = int(p.T / p.dt)
     _, r_c, th = teuk.grid(p)
     rows = workloads.plunge_rows(p, nt, eta=1e-3, iota_deg=30.0, seed=0)
     pts = p.N * p.M
-    log(f"bench: workload single_16384x64_plunge N={p.N} M={p.M} warmup={args.warmup} steps={args.steps}")
+    log(f"bench: workload {args.workload} N={p.N} M={p.M} warmup={args.warmup} steps={args.steps}")
     with teuk.Solver(p, r_c, th) as s:
         if args.wpb or args.wpe:
             s.set_tiling(args.wpb or 4, args.wpe or 2)
@@ -215,7 +228,7 @@ def main():
     # B_alg is per full step; a step is STAGES stage launches
     b_alg = B_ALG[args.integrator]
     achieved = (b_alg / STAGES[args.integrator]) * pts / per_launch_s / 1e9
-    traffic = pmc_traffic(args.integrator)
+    traffic = pmc_traffic(args.integrator) if args.workload == "single_16384x64_plunge" else None
     out = {
         "metric": "FP64 point-updates/s (%HBM roofline)",
         "value": value,
@@ -229,7 +242,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "single_16384x64_plunge", "N": p.N, "M": p.M, "rho": [p.xmin, p.xmax],
+        "config": {"workload": args.workload, "N": p.N, "M": p.M, "rho": [p.xmin, p.xmax],
                    "a": p.a, "m": p.m, "dt_over_drho": 0.5,
                    "integrator": {"rk4": "classic RK4", "rk3": "SSP-RK3"}[args.integrator],
                    "t_final": p.T,

@@ -55,6 +55,21 @@ def single_16384x64(T=1500.0, integrator=0, Fr=2):
     return p
 
 
+HIGH_SPIN_GRIDS = ((16384, 32), (32768, 64), (65536, 128))
+
+
+def high_spin(N, M, T=2000.0, integrator=1, Fr=2):
+    """High-spin resolution ladder: s=-2, m=2, a=0.99 on rho in [-600,1600]
+    (layer from rho=808), dt = 0.5 drho to t=2000; one of HIGH_SPIN_GRIDS
+    (16384x32: 29789 steps; 32768x64: 59578; 65536x128: 119156, 8.4M points).
+    Psi at rho=1400."""
+    p = teuk.default_params(N, M, -600.0, 1600.0, 0.99, 2, dt_over_dx=0.5, T=T)
+    p.Fr = Fr
+    p.integrator = integrator
+    p.j_extract = j_at(p, 1400.0)
+    return p
+
+
 def plunge_rows(p, nsteps, eta=1e-3, iota_deg=30.0, seed=0, r0_over_isco=1.02):
     """Seeded transition-plus-plunge .traj rows for spin p.a, sampled at dt/2
     and long enough for nsteps of the reference's two-rows-per-step reading."""

@@ -83,8 +83,12 @@ def lib(timing=False):
     return _libs[name]
 
 
+# Threads for the point loops (results do not depend on the count).
+DEFAULT_THREADS = max(1, min(16, os.cpu_count() or 1))
+
+
 def make_params(N, M, xmin, xmax, a, m, dt=None, mass=1.0, integrator=0, j_extract=-1,
-                ptsx=128, ptsy=128, nthreads=1, dt_over_dx=0.5):
+                ptsx=128, ptsy=128, nthreads=DEFAULT_THREADS, dt_over_dx=0.5):
     dx = (xmax - xmin) / N
     return Params(N, M, xmin, xmax, mass, a, -2.0, float(m), dx, PI15 / M,
                   dt if dt is not None else dt_over_dx * dx,

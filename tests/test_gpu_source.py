@@ -8,7 +8,7 @@ import math
 import numpy as np
 import pytest
 
-from bh_ringdown_amd import teuk
+from bh_ringdown_amd import teuk, workloads
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -149,3 +149,60 @@ def test_patch_at_domain_edges(steps):
     print(f"edge case steps={steps}", flush=True)
     ug, uo, *_ = run_both(p, steps, rows)
     assert_same_bits(ug, uo)
+
+
+@pytest.mark.parametrize("N,M", workloads.HIGH_SPIN_GRIDS)
+def test_high_spin_ladder_full_size(N, M):
+    """The a=0.99 resolution ladder on rho in [-600,1600] at its full grid
+    sizes (up to 65536 x 128), classic RK4, a Gaussian pulse plus a seeded
+    plunge source: 8 steps against the oracle, bit for bit."""
+    steps = 8
+    p = workloads.high_spin(N, M)
+    p.T = steps * p.dt * (1 + 1e-12)
+    rows = workloads.plunge_rows(p, steps, seed=4)
+    _, r_c, th = teuk.grid(p)
+    u0 = workloads.gaussian_pulse(p, seed=6, center=200.0, width=10.0)
+    print(f"high-spin case N={N} M={M} steps={steps}", flush=True)
+    with teuk.Solver(p, r_c, th) as s:
+        s.set_fields(u0)
+        s.set_trajectory(rows, steps)
+        s.step(steps)
+        s.sync()
+        ug, out, psig = s.get_fields(), s.output(), s.psi()
+    sched = teuk.trajectory_schedule(rows, steps, p.dt)
+    op = O.make_params(p.N, p.M, p.xmin, p.xmax, p.a, p.m, dt=p.dt, integrator=p.integrator,
+                       j_extract=p.j_extract)
+    ost = [O.SrcStep(*[getattr(x, f) for f, _ in teuk.SourceStep._fields_]) for x in sched]
+    uo = u0.copy()
+    rc, psio, ho, _ = O.run(op, r_c, th, uo, steps, src_steps=ost)
+    assert rc == 0
+    assert_same_bits(ug, uo)
+    assert np.array_equal(psig, psio[out["k"]])
+    assert np.array_equal(out["h"], ho[out["k"]])
+
+
+@pytest.mark.parametrize("N,M", workloads.HIGH_SPIN_GRIDS[-1:])
+def test_high_spin_affine_at_full_size(N, M):
+    """With the source on, one step is an affine map u -> A u + s, so
+    F(u1 + 2 u2) + 2 F(0) = F(u1) + 2 F(u2) after any number of steps; checked
+    at 65536 x 128 over 40 RK4 steps."""
+    steps = 40
+    p = workloads.high_spin(N, M)
+    p.T = steps * p.dt * (1 + 1e-12)
+    rows = workloads.plunge_rows(p, steps, seed=4)
+    _, r_c, th = teuk.grid(p)
+    u1 = workloads.gaussian_pulse(p, seed=21, center=100.0, width=10.0)
+    u2 = workloads.gaussian_pulse(p, seed=22, center=600.0, width=10.0)
+
+    def F(u):
+        with teuk.Solver(p, r_c, th) as s:
+            s.set_fields(u)
+            s.set_trajectory(rows, steps)
+            s.step(steps)
+            s.sync()
+            return s.get_fields()
+
+    print(f"high-spin affine case N={N} M={M} steps={steps}", flush=True)
+    lhs = F(u1 + 2.0 * u2) + 2.0 * F(np.zeros_like(u1))
+    rhs = F(u1) + 2.0 * F(u2)
+    assert np.linalg.norm(lhs - rhs) <= 1e-13 * np.linalg.norm(rhs)

@@ -245,7 +245,7 @@ def test_oracle_timing_build_matches_parity_build():
     p = teuk.default_params(256, 32, -50.0, 50.0, 0.5, 2)
     _, r, th = teuk.grid(p)
     u0 = workloads.gaussian_pulse(p, 3, center=10.0)
-    op = O.make_params(p.N, p.M, p.xmin, p.xmax, p.a, p.m, dt=p.dt)
+    op = O.make_params(p.N, p.M, p.xmin, p.xmax, p.a, p.m, dt=p.dt, nthreads=1)
     u1 = u0.copy()
     O.run(op, r, th, u1, 50)
     op.nthreads = 4
@@ -254,3 +254,27 @@ def test_oracle_timing_build_matches_parity_build():
     O.lib(timing=True).orc_run(O.ctypes.byref(op), O._ptr(r), O._ptr(th), O._ptr(u2), 50, None,
                                O._ptr(psi), None, O._ptr(dedt))
     assert np.linalg.norm(u2 - u1) / np.linalg.norm(u1) < 1e-12
+
+
+@pytest.mark.parametrize("integrator", [0, 1])
+def test_parity_build_thread_count_does_not_change_bits(integrator):
+    """The parity build splits only point loops over threads: 1 and 5 threads
+    give the same bits, with the source on."""
+    from bh_ringdown_amd import teuk, workloads
+    p = teuk.default_params(256, 32, -40.0, 60.0, 0.7, 2)
+    steps = 30
+    _, r, th = teuk.grid(p)
+    u0 = workloads.gaussian_pulse(p, 5, center=10.0)
+    rows = teuk.plunge_trajectory(p.a, 1e-2, 0.4, p.dt / 2, (2 * steps + 6) * p.dt / 2, seed=2,
+                                  r0_over_isco=1.3)
+    ost = [O.SrcStep(*[getattr(x, f) for f, _ in teuk.SourceStep._fields_])
+           for x in teuk.trajectory_schedule(rows, steps, p.dt)]
+    got = []
+    for nt in (1, 5):
+        op = O.make_params(p.N, p.M, p.xmin, p.xmax, p.a, p.m, dt=p.dt, integrator=integrator, nthreads=nt)
+        u = u0.copy()
+        rc, psi, h, _ = O.run(op, r, th, u, steps, src_steps=ost)
+        assert rc == 0
+        got.append((u, psi, h))
+    for a, b in zip(*got):
+        assert np.array_equal(a, b)

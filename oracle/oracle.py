@@ -70,6 +70,12 @@ def _load(name):
     lib.orc_t4_coeffs.argtypes = [_dp, _dp]
     lib.orc_exp_neg.argtypes = [ctypes.c_double]
     lib.orc_exp_neg.restype = ctypes.c_double
+    lib.orc_grid_half.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, _dp, _dp]
+    lib.orc_coeffs_so2.argtypes = [P, _dp, _dp, _dp, _dp, _dp]
+    lib.orc_step_so2.argtypes = [P, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    lib.orc_run_so2.argtypes = [P, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.POINTER(SrcStep), _dp, _dp, _dp]
+    lib.orc_run_so2.restype = ctypes.c_int
     return lib
 
 
@@ -173,6 +179,47 @@ def run(p, r_c, theta, u, nsteps, src_steps=None, timing=False, want_h=True):
         steps = ctypes.cast(arr, ctypes.POINTER(SrcStep))
     rc = lib(timing).orc_run(ctypes.byref(p), _ptr(r_c), _ptr(theta), _ptr(u), nsteps, steps,
                              _ptr(psi), _ptr(h) if h is not None else None, _ptr(dedt))
+    return rc, psi, h, dedt
+
+
+# ---------------------------------------------- staggered scheme (Old/) --
+def grid_half(N, xmin, xmax, mass, a):
+    """Half-point rho_h = Xmin + (l - 1/2) dx and radii r_h of the staggered scheme."""
+    x = np.zeros(N); r = np.zeros(N)
+    lib().orc_grid_half(N, xmin, xmax, mass, a, _ptr(x), _ptr(r))
+    return x, r
+
+
+def coeffs_so2(p, theta, r_h, r_c):
+    ch = np.zeros((9, p.M, p.N)); cc = np.zeros((9, p.M, p.N))
+    lib().orc_coeffs_so2(ctypes.byref(p), _ptr(theta), _ptr(r_h), _ptr(r_c), _ptr(ch), _ptr(cc))
+    return ch, cc
+
+
+def step_so2(p, coef_h, coef_c, theta, u, src_h=None, src_c=None, work=None):
+    """One staggered step in place; pass the same work (12 planes, zero at
+    first) from step to step."""
+    if work is None:
+        work = np.zeros((12, p.M, p.N))
+    lib().orc_step_so2(ctypes.byref(p), _ptr(coef_h), _ptr(coef_c), _ptr(theta), _ptr(u),
+                       _ptr(src_h) if src_h is not None else None,
+                       _ptr(src_c) if src_c is not None else None, _ptr(work))
+    return u
+
+
+def run_so2(p, r_h, r_c, theta, u, nsteps, src_steps=None, want_h=True):
+    """Staggered-scheme run (two source entries per step); as run()."""
+    M = p.M
+    psi = np.zeros((nsteps, 2, M))
+    h = np.zeros((nsteps, 14)) if want_h else None
+    dedt = np.zeros(nsteps)
+    steps = None
+    if src_steps is not None:
+        assert len(src_steps) >= 2 * nsteps
+        arr = (SrcStep * len(src_steps))(*src_steps)
+        steps = ctypes.cast(arr, ctypes.POINTER(SrcStep))
+    rc = lib().orc_run_so2(ctypes.byref(p), _ptr(r_h), _ptr(r_c), _ptr(theta), _ptr(u), nsteps, steps,
+                           _ptr(psi), _ptr(h) if h is not None else None, _ptr(dedt))
     return rc, psi, h, dedt
 
 

@@ -28,6 +28,8 @@
  *   orc_source    parity unpinned against the reference (kern.cu needs the
  *                 CUDA compiler, absent here); pinned by the distributional
  *                 canonical form and physical checks (see teuk_source.inc).
+ *   orc_*_so2     the staggered second-order scheme of TD_GPU_code/Old
+ *                 (teuk_oracle_so2.inc lists its checks).
  */
 #include "teuk_oracle.h"
 #include "teuk_point.h"
@@ -58,32 +60,26 @@ inline int extract_col(const orc_params *p) {
 
 }  // namespace
 
-extern "C" {
-
 /* ---------------------------------------------------------------- grid --- */
-/* main.c:79-198: uniform rho and theta grids, the hyperboloidal layer Omega,
+namespace {
+
+/* main.c:79-198 from the rho grid on: the hyperboloidal layer Omega,
  * r* = rho/Omega, and a Newton solve for the Boyer-Lindquist r(r*) marching
  * from the outer edge inward, each point warm-started from its neighbour. */
-void orc_grid(int N, int M, double xmin, double xmax, double mass, double a,
-              double *x_c, double *r_c, double *theta) {
-  const double dx = (xmax - xmin) / N;
-  const double dth = kPi15 / M;
-  for (int l = 0; l < N; l++) x_c[l] = xmin + (double)l * dx;
-  for (int k = 0; k < M; k++) theta[k] = (double)k * dth;
-
+void newton_radii(int N, const double *x, double xmin, double xmax, double mass, double a, double *r_out) {
   const double rin = xmin + (xmax - xmin) * 0.64;
   const double outer = xmax + 1e-6;
   std::vector<double> rstar(N);
   for (int l = 0; l < N; l++) {
     double om = 1.0;
-    if (!(x_c[l] < rin)) om = 1.0 - std::pow((x_c[l] - rin) / (outer - rin), 4);
-    rstar[l] = x_c[l] / om;
+    if (!(x[l] < rin)) om = 1.0 - std::pow((x[l] - rin) / (outer - rin), 4);
+    rstar[l] = x[l] / om;
   }
 
   const double root = std::sqrt(mass * mass - a * a);
   const double rplus = mass + root, rminus = mass - root;
   if (mass == 0.0) {
-    for (int l = 0; l < N; l++) r_c[l] = rstar[l];
+    for (int l = 0; l < N; l++) r_out[l] = rstar[l];
     return;
   }
   const double tol = 1e-12;
@@ -112,34 +108,47 @@ void orc_grid(int N, int M, double xmin, double xmax, double mass, double a,
       rel = std::fabs(rnext - r) / r;
       r = rnext;
     } while (rel > tol);
-    r_c[l] = rnext;
+    r_out[l] = rnext;
   }
 }
 
+}  // namespace
+
+extern "C" {
+
+/* main.c:79-198 grid(): uniform rho and theta grids and the radii. */
+void orc_grid(int N, int M, double xmin, double xmax, double mass, double a,
+              double *x_c, double *r_c, double *theta) {
+  const double dx = (xmax - xmin) / N;
+  const double dth = kPi15 / M;
+  for (int l = 0; l < N; l++) x_c[l] = xmin + (double)l * dx;
+  for (int k = 0; k < M; k++) theta[k] = (double)k * dth;
+  newton_radii(N, x_c, xmin, xmax, mass, a, r_c);
+}
+
 /* ---------------------------------------------------------- coefficients --- */
-/* kern.cu:84-169.  The compactified Teukolsky operator is written as
- *   d_tau^2 Psi = A^tr d_tau d_rho Psi + A^rr d_rho^2 Psi
- *               + A^yy (d_th^2 + cot d_th) Psi + B^tau d_tau Psi
- *               + B^rho d_rho Psi + C Psi
- * and reduced to first order with Pi = d_tau Psi + bb d_rho Psi (kern.cu:147-158). */
-void orc_coeffs(const orc_params *p, const double *theta, const double *r_c,
-                double *coef) {
-  const int N = p->N, M = p->M;
-  const size_t NM = plane(p);
-  std::memset(coef, 0, 9 * NM * sizeof(double));
+}  // extern "C"
+
+namespace {
+
+/* The theta dependence of kernel_init's formulas (kern.cu:132-145) enters
+ * only through sin^2(theta) in bir, cos(theta) in B^tau_im and the angular
+ * potential Y = 4 cot^2 - 4 m cot csc + m^2 csc^2 in C_re.  Each field is
+ * rewritten as (column constant [+ column constant x row value]) / bir, with
+ * bir = K2 sin^2 - K1; the algebra is that of kern.cu:132-145 term by term,
+ * only regrouped.  Column j sits at rho = Xmin + (j + shift) dx with radius
+ * r[j] (shift 0: the grid points; -1/2: the half points of the staggered
+ * scheme, Old/kern.cu:82-83). */
+enum { K1, K2, KA, KR, KD, KBt, KBr, KC0, KI0, KI1, KBi, KCi, NK };   /* teuk_point.h order */
+
+void column_constants(const orc_params *p, double shift, const double *r_col, std::vector<double> &K) {
+  const int N = p->N;
   const double mass = p->mass, spin = p->a, em = p->m;
   const double rin = p->xmin + (p->xmax - p->xmin) * 0.64;
   const double outer = p->xmax + 1e-6;
-  /* The theta dependence of kernel_init's formulas enters only through
-   * sin^2(theta) in bir, cos(theta) in B^tau_im and the angular potential
-   * Y = 4 cot^2 - 4 m cot csc + m^2 csc^2 in C_re.  Each field is rewritten
-   * as (column constant [+ column constant x row value]) / bir, with
-   * bir = K2 sin^2 - K1; the algebra is that of kern.cu:132-145 term by
-   * term, only regrouped. */
-  enum { K1, K2, KA, KR, KD, KBt, KBr, KC0, KI0, KI1, KBi, KCi, NK };   /* teuk_point.h order */
-  std::vector<double> K((size_t)NK * N);
+  K.assign((size_t)NK * N, 0.0);
   for (int j = 0; j < N; j++) {
-    const double x = p->xmin + (double)j * p->dx;
+    const double x = p->xmin + ((double)j + shift) * p->dx;
     double om, om1, om2;
     if (x < rin) {
       om = 1.0; om1 = 0.0; om2 = 0.0;
@@ -149,7 +158,7 @@ void orc_coeffs(const orc_params *p, const double *theta, const double *r_c,
       om2 = -12.0 * std::pow(x - rin, 2) / std::pow(outer - rin, 4);
     }
     const double L = om - x * om1;      /* Omega^2 times d(r*)/d(rho) */
-    const double R = om * r_c[j];       /* compactified radius */
+    const double R = om * r_col[j];     /* compactified radius */
     const double a2 = spin * spin, a4 = a2 * a2;
     const double om_2 = om * om, om3 = om_2 * om, om4 = om_2 * om_2, om5 = om4 * om;
     const double S = R * R + om_2 * a2;                 /* (r^2+a^2) Omega^2 */
@@ -182,7 +191,17 @@ void orc_coeffs(const orc_params *p, const double *theta, const double *r_c,
     c[KBi * N + j] = (-2.0 * em * om_2 * spin * S) / L;
     c[KCi * N + j] = (2.0 * em * om * spin * (R2 + 2.0 * R * mass * om - 3.0 * om_2 * a2)) / R;
   }
-  for (int k = 3; k < M - 3; k++) {
+}
+
+/* The nine fields (kern.cu:147-158) on rows lo <= k < hi from the column
+ * constants; zero elsewhere (kern.cu:113). */
+void fill_coeffs(const orc_params *p, const double *theta, const std::vector<double> &K, int lo, int hi,
+                 double *coef) {
+  const int N = p->N;
+  const size_t NM = plane(p);
+  const double em = p->m;
+  std::memset(coef, 0, 9 * NM * sizeof(double));
+  for (int k = lo; k < hi; k++) {
     const double y = theta[k];
     const double sn = std::sin(y), cs = std::cos(y);
     const double cot = 1.0 / std::tan(y), csc = 1.0 / std::sin(y);
@@ -204,6 +223,23 @@ void orc_coeffs(const orc_params *p, const double *theta, const double *r_c,
       coef[8 * NM + id] = cf.vim;
     }
   }
+}
+
+}  // namespace
+
+extern "C" {
+
+/* kern.cu:84-169.  The compactified Teukolsky operator is written as
+ *   d_tau^2 Psi = A^tr d_tau d_rho Psi + A^rr d_rho^2 Psi
+ *               + A^yy (d_th^2 + cot d_th) Psi + B^tau d_tau Psi
+ *               + B^rho d_rho Psi + C Psi
+ * and reduced to first order with Pi = d_tau Psi + bb d_rho Psi (kern.cu:147-158);
+ * the fields are set on rows 3 <= k < M-3 (kern.cu:113). */
+void orc_coeffs(const orc_params *p, const double *theta, const double *r_c,
+                double *coef) {
+  std::vector<double> K;
+  column_constants(p, 0.0, r_c, K);
+  fill_coeffs(p, theta, K, 3, p->M - 3, coef);
 }
 
 /* ------------------------------------------------------------------- rhs --- */
@@ -460,3 +496,4 @@ int orc_run(const orc_params *p, const double *r_c, const double *theta,
 }  // extern "C"
 
 #include "teuk_source.inc"
+#include "teuk_oracle_so2.inc"

@@ -26,7 +26,8 @@ typedef struct orc_params {
   double xmin, xmax;   /* parm.h:3-4 (rho domain) */
   double mass, a, s, m;/* main.c:55-57, body() mm */
   double dx, dtheta, dt;
-  int integrator;      /* 0 = SSP-RK3 (reference), 1 = classic RK4 */
+  int integrator;      /* 0 = SSP-RK3 (reference), 1 = classic RK4,
+                          2 = staggered 2nd order (TD_GPU_code/Old, orc_*_so2) */
   int j_extract;       /* extraction column, -1 = N-1 (kern.cu:750-751) */
   int ptsx, ptsy;      /* parm.h:10-11 source patch size */
   int nthreads;        /* OpenMP threads in the timing build (<= 1: serial) */
@@ -80,6 +81,18 @@ double orc_project(double *qq, int M, double l, double m);
 int orc_run(const orc_params *p, const double *r_c, const double *theta,
             double *u, int nsteps, const orc_src_step *src_steps,
             double *psi_hist, double *h_hist, double *dedt_hist);
+
+/* Staggered second-order scheme of TD_GPU_code/Old (teuk_oracle_so2.inc):
+ * half-point radii, the two coefficient sets (9 planes each), one step
+ * (work: 12 planes, zeroed before the first step and kept between steps),
+ * and a run with two source entries per step (half point, then grid point). */
+void orc_grid_half(int N, double xmin, double xmax, double mass, double a, double *x_h, double *r_h);
+void orc_coeffs_so2(const orc_params *p, const double *theta, const double *r_h, const double *r_c,
+                    double *coef_h, double *coef_c);
+void orc_step_so2(const orc_params *p, const double *coef_h, const double *coef_c, const double *theta,
+                  double *u, const double *src_h, const double *src_c, double *work);
+int orc_run_so2(const orc_params *p, const double *r_h, const double *r_c, const double *theta, double *u,
+                int nsteps, const orc_src_step *src_steps, double *psi_hist, double *h_hist, double *dedt_hist);
 
 #ifdef __cplusplus
 }

@@ -123,4 +123,19 @@ TEUK_HD double rk3_s3(double base, double u, double dt, double r, double third) 
 // u + dt/6 L1 + dt/3 L2 + dt/3 L3, result acc + dt/6 L4.
 TEUK_HD double axpy(double h, double r, double y) { return fma_(h, r, y); }
 
+// Staggered second-order scheme of TD_GPU_code/Old/kern.cu: fields averaged
+// to the half points rho_{a-1/2} (kernel_average1, :196-212) and back
+// (kernel_average2, :346-362), centred theta differences (:245-254,
+// :395-404), one-cell rho differences (:260-263, :410-413), the 2nd-order
+// dbdx (:231-233, :381-383), half and full updates (:297-300, :448-451) and
+// the theta/rho edge extrapolation (:305-340, :456-522), which divides by 3.
+TEUK_HD double so2_avg(double f0, double f1) { return 0.5 * (f0 + f1); }
+TEUK_HD double so2_dth(double fp, double fm, double dth) { return (fp - fm) / (2.0 * dth); }
+TEUK_HD double so2_dth2(double f0, double fp, double fm, double dth) {
+  return (fp + fm - 2.0 * f0) / (dth * dth);
+}
+TEUK_HD double so2_dx(double f1, double f0, double dx) { return (f1 - f0) / dx; }
+TEUK_HD double so2_dbdx(double bp, double bm, double dx) { return (bp - bm) / dx / 2.0; }
+TEUK_HD double so2_edge(double f1, double f2) { return (4.0 * f1 - f2) / 3.0; }
+
 }  // namespace teuk_pt

@@ -22,7 +22,7 @@ HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contrac
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wno-unused-result", f"-I{INCLUDE}", f"-I{CSRC}"]
 
-LIB_SOURCES = ["teuk_kernels.hip", "teuk_api.hip", "teuk_host.cpp", "teuk_traj.cpp"]
+LIB_SOURCES = ["teuk_kernels.hip", "teuk_so2.hip", "teuk_api.hip", "teuk_host.cpp", "teuk_traj.cpp"]
 
 
 def _run(cmd):

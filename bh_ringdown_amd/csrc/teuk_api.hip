@@ -91,7 +91,7 @@ int validate(const teuk_params *p) {
   if (!(p->dt > 0.0) || !(p->dx > 0.0) || !(p->dtheta > 0.0)) return TEUK_EINVAL;
   if (!(p->xmax > p->xmin)) return TEUK_EINVAL;
   if (p->Fr < 1 || p->ptsx < 1 || p->ptsy < 1) return TEUK_EINVAL;
-  if (p->integrator != 0 && p->integrator != 1) return TEUK_EINVAL;
+  if (p->integrator < 0 || p->integrator > 2) return TEUK_EINVAL;
   if (p->j_extract >= p->N || p->j_extract < -1) return TEUK_EINVAL;
   if (!(std::fabs(p->a) < p->mass)) return TEUK_EINVAL;  // horizon formulas need |a| < M
   if (p->m != std::floor(p->m)) return TEUK_EINVAL;
@@ -101,6 +101,14 @@ int validate(const teuk_params *p) {
 
 int scheduled_steps(const teuk_params *p) { return (int)(p->T / p->dt); }
 
+// Kernel launches per step that the timing events cover as "stages": the
+// RK stages, or the staggered scheme's half and full step.
+int stages_per_step(int integrator) { return integrator == 0 ? 3 : (integrator == 1 ? 4 : 2); }
+
+// Steps the source schedule covers (two entries per step for the staggered
+// scheme).
+int sched_steps(const teuk_solver *s) { return s->g.stag ? s->d.nsched / 2 : s->d.nsched; }
+
 void flush_timing(teuk_solver *s) {
   if (s->ev_used == 0) return;
   (void)hipStreamSynchronize(s->stream);
@@ -108,7 +116,7 @@ void flush_timing(teuk_solver *s) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, s->ev[i], s->ev[i + 1]) == hipSuccess) {
       s->timed_ms += ms;
-      s->timed_launches += s->P.integrator == 0 ? 3 : 4;
+      s->timed_launches += stages_per_step(s->P.integrator);
     }
   }
   s->ev_used = 0;
@@ -118,10 +126,12 @@ void flush_timing(teuk_solver *s) {
 // (every kRing steps, for the next kRing steps), then the fused stages.
 // Output records are completed by launch_finalize once per batch of steps.
 int enqueue_step(teuk_solver *s, int k_rel) {
-  const int nstage = s->P.integrator == 0 ? 3 : 4;
+  const int nstage = stages_per_step(s->P.integrator);
   // the patches of this step and the next ones (up to the ring size) come
-  // from one launch at the first step of each group of kRing steps
-  if (s->has_src && k_rel % kRing == 0) HIPCHK(launch_source(s->g, s->d, k_rel, kRing, s->stream));
+  // from one launch at the first step of each group of kRing steps (kRing
+  // half steps for the staggered scheme)
+  const int group = s->g.stag ? kRing / 2 : kRing;
+  if (s->has_src && k_rel % group == 0) HIPCHK(launch_source(s->g, s->d, k_rel, kRing, s->stream));
   // Timing: one event pair around the step's back-to-back stage launches
   // (an event between two launches would add its own gap to both).
   if (s->timing) {
@@ -138,8 +148,12 @@ int enqueue_step(teuk_solver *s, int k_rel) {
     }
     HIPCHK(hipEventRecord(s->ev[s->ev_used], s->stream));
   }
-  for (int st = 1; st <= nstage; st++)
-    HIPCHK(launch_stage(s->g, s->d, s->P.integrator, st, s->has_src, k_rel, s->tiling, s->stream));
+  if (s->g.stag) {
+    HIPCHK(launch_so2_step(s->g, s->d, s->has_src, k_rel, s->stream));
+  } else {
+    for (int st = 1; st <= nstage; st++)
+      HIPCHK(launch_stage(s->g, s->d, s->P.integrator, st, s->has_src, k_rel, s->tiling, s->stream));
+  }
   if (s->timing) {
     HIPCHK(hipEventRecord(s->ev[s->ev_used + 1], s->stream));
     s->ev_used += 2;
@@ -217,6 +231,58 @@ int consume_trajectory(const double *rows, int nrows, int nsteps, double start, 
   return TEUK_OK;
 }
 
+// Old/body.cu:113-180, the staggered scheme's reading of the same stream:
+// the same priming, then one row per half step, each used as read, with
+// r += 1e-6 and accelerations (v - v_prev) / dt * 2 over the half step; both
+// entries of a step carry the step's timer.
+int consume_trajectory_half(const double *rows, int nrows, int nsteps, double start, double dt,
+                            std::vector<SrcStep> &out) {
+  out.clear();
+  out.reserve(2 * (size_t)nsteps);
+  int cur = 0;
+  auto read = [&](const double *&row) {
+    if (cur >= nrows) return false;
+    row = rows + (size_t)10 * cur++;
+    return true;
+  };
+  const double *row = nullptr;
+  if (!read(row)) return TEUK_ERANGE;
+  do {
+    if (!read(row)) return TEUK_ERANGE;
+  } while (row[0] < start);
+  double v_r = 0.0, v_th = 0.0, v_ph = 0.0;
+  double timer = start;
+  for (int k = 0; k < nsteps; k++) {
+    for (int h = 0; h < 2; h++) {
+      if (!read(row)) return TEUK_ERANGE;
+      SrcStep st;
+      st.timer = timer;
+      st.start = start;
+      st.rp = row[1] + 1e-6;
+      st.tp = row[2];
+      st.drdt = row[3];
+      st.dthdt = row[4];
+      st.dphidt = row[5];
+      st.phip = row[6];
+      st.E = row[7];
+      st.lz = row[8];
+      st.Q = row[9];
+      st.d2rdt2 = (row[3] - v_r) / dt * 2.0;
+      st.d2thdt2 = (row[4] - v_th) / dt * 2.0;
+      st.d2phidt2 = (row[5] - v_ph) / dt * 2.0;
+      v_r = row[3];
+      v_th = row[4];
+      v_ph = row[5];
+      out.push_back(st);
+    }
+    timer += dt;
+  }
+  return TEUK_OK;
+}
+
+int create_solver(const teuk_params *p, const double *r_h, const double *r_c, const double *theta,
+                  teuk_solver **out);
+
 int read_traj_file(const char *path, std::vector<double> &rows) {
   FILE *f = std::fopen(path, "r");
   if (!f) return TEUK_EIO;
@@ -275,16 +341,24 @@ int teuk_grid(const teuk_params *p, double *x_c, double *r_c, double *theta) {
   return TEUK_OK;
 }
 
+int teuk_grid_half(const teuk_params *p, double *x_h, double *r_h) {
+  if (!p || !x_h || !r_h || p->N <= 0) return TEUK_EINVAL;
+  host_grid_half(p->N, p->xmin, p->xmax, p->mass, p->a, x_h, r_h);
+  return TEUK_OK;
+}
+
 long long teuk_footprint(const teuk_params *p, int nsteps) {
   if (!p) return -1;
   const long long NM = (long long)p->N * p->M;
-  const long long states = (p->integrator == 0 ? 3 : 4) * 4 * NM;
+  const long long states = (p->integrator == 0 ? 3 : (p->integrator == 1 ? 4 : 2)) * 4 * NM;
+  const long long entries = (long long)std::max(nsteps, 0) * (p->integrator == 2 ? 2 : 1);
   const long long nrec = (long long)(std::max(nsteps, 0) / std::max(p->Fr, 1)) + 1;
   long long bytes = 8 * (states + 2 * NM + (kNK + 1) * (long long)p->N + (kNR + 1) * (long long)p->M);
   bytes += 8 * ((long long)kRing * 2 * p->ptsx * p->ptsy) + 4LL * 2 * kRing;
   bytes += 8 * (4LL * p->M + 2 * p->M + 1 + 7LL * 5 * (p->M / 4) * 3);
   bytes += nrec * (16 * 8 + 2 * 4 + 6LL * p->M * 8);
-  bytes += (long long)nsteps * (long long)(sizeof(SrcDev) + sizeof(teuk_t4::StepIn));
+  bytes += entries * (long long)(sizeof(SrcDev) + sizeof(teuk_t4::StepIn));
+  if (p->integrator == 2) bytes += 8 * (NM + (kNK + 3) * (long long)p->N);  // dbdx_h, ktab_h, r_h, phase_h
   return bytes;
 }
 
@@ -293,6 +367,33 @@ int teuk_create(const teuk_params *p, const double *r_c, const double *theta, te
   *out = nullptr;
   int rc = validate(p);
   if (rc) return rc;
+  if (p->integrator == 2) {
+    std::vector<double> x_h(p->N), r_h(p->N);
+    host_grid_half(p->N, p->xmin, p->xmax, p->mass, p->a, x_h.data(), r_h.data());
+    return teuk_create_staggered(p, r_h.data(), r_c, theta, out);
+  }
+  return create_solver(p, nullptr, r_c, theta, out);
+}
+
+int teuk_create_staggered(const teuk_params *p, const double *r_h, const double *r_c, const double *theta,
+                          teuk_solver **out) {
+  if (!out || !r_h || !r_c || !theta) return TEUK_EINVAL;
+  *out = nullptr;
+  int rc = validate(p);
+  if (rc) return rc;
+  if (p->integrator != 2) return TEUK_EINVAL;
+  return create_solver(p, r_h, r_c, theta, out);
+}
+
+}  // extern "C"
+
+namespace {
+
+// Device state, tables and coefficient set-up of one solver; r_h only for
+// the staggered scheme.
+int create_solver(const teuk_params *p, const double *r_h, const double *r_c, const double *theta,
+                  teuk_solver **out) {
+  int rc = TEUK_OK;
   teuk_solver *s = new teuk_solver();
   s->P = *p;
   s->nt = scheduled_steps(p);
@@ -315,6 +416,7 @@ int teuk_create(const teuk_params *p, const double *r_c, const double *theta, te
   g.a = p->a;
   g.m = p->m;
   g.start = 0.0;
+  g.stag = p->integrator == 2 ? 1 : 0;
 
   auto fail = [&](int code) {
     teuk_destroy(s);
@@ -323,12 +425,12 @@ int teuk_create(const teuk_params *p, const double *r_c, const double *theta, te
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return fail(TEUK_EHIP);
   const size_t NM = (size_t)p->N * p->M;
   DevState &d = s->d;
-  const int nstates = p->integrator == 0 ? 3 : 4;
+  const int nstates = p->integrator == 0 ? 3 : (p->integrator == 1 ? 4 : 2);
   double *states = nullptr;
   if ((rc = dalloc(s, &states, nstates * 4 * NM))) return fail(rc);
   d.U = states;
-  d.C = states + 4 * NM;
-  d.D = states + 8 * NM;
+  d.C = states + 4 * NM;   // staggered scheme: the half-point fields
+  d.D = nstates >= 3 ? states + 8 * NM : nullptr;
   d.A = nstates == 4 ? states + 12 * NM : nullptr;
   if ((rc = dalloc(s, &d.ktab, kNK * (size_t)p->N))) return fail(rc);
   if ((rc = dalloc(s, &d.rtab, kNR * (size_t)p->M))) return fail(rc);
@@ -350,6 +452,12 @@ int teuk_create(const teuk_params *p, const double *r_c, const double *theta, te
   d.ext.projw = projw;
   if ((rc = dalloc(s, &d.ctl, 1))) return fail(rc);
   if ((rc = dalloc(s, &d.phase, 2 * (size_t)p->N))) return fail(rc);
+  if (g.stag) {
+    if ((rc = dalloc(s, &d.ktab_h, kNK * (size_t)p->N))) return fail(rc);
+    if ((rc = dalloc(s, &d.dbdx_h, NM))) return fail(rc);
+    if ((rc = dalloc(s, &d.r_h, p->N))) return fail(rc);
+    if ((rc = dalloc(s, &d.phase_h, 2 * (size_t)p->N))) return fail(rc);
+  }
   d.sched = nullptr;
   d.nsched = 0;
   {
@@ -357,6 +465,12 @@ int teuk_create(const teuk_params *p, const double *r_c, const double *theta, te
     host_source_phase(p->N, p->mass, p->a, p->m, r_c, ph.data());
     if (hipMemcpy(d.phase, ph.data(), ph.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
       return fail(TEUK_EHIP);
+    if (g.stag) {
+      host_source_phase(p->N, p->mass, p->a, p->m, r_h, ph.data());
+      if (hipMemcpy(d.phase_h, ph.data(), ph.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(d.r_h, r_h, p->N * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(TEUK_EHIP);
+    }
   }
 
   // Zero fields and integrals (kern.cu:164-167, body.cu:86-91).
@@ -392,7 +506,17 @@ int teuk_create(const teuk_params *p, const double *r_c, const double *theta, te
   h_extw[2 * p->M] = std::pow(rr, 6);
   std::vector<double> ktab(kNK * (size_t)p->N), rtab(kNR * (size_t)p->M);
   host_coeff_tables(p->N, p->M, p->xmin, p->xmax, p->dx, p->mass, p->a, p->m, r_c, theta, ktab.data(),
-                    rtab.data());
+                    rtab.data(), 0.0, g.stag != 0);
+  std::vector<double> ktab_h, rtab_h;
+  if (g.stag) {  // Old/kern.cu:90-135: the same fields at rho - dx/2 with r_h
+    ktab_h.resize(ktab.size());
+    rtab_h.resize(rtab.size());
+    host_coeff_tables(p->N, p->M, p->xmin, p->xmax, p->dx, p->mass, p->a, p->m, r_h, theta, ktab_h.data(),
+                      rtab_h.data(), -0.5, true);
+    if (hipMemcpyAsync(d.ktab_h, ktab_h.data(), ktab_h.size() * sizeof(double), hipMemcpyHostToDevice,
+                       s->stream) != hipSuccess)
+      return fail(TEUK_EHIP);
+  }
   std::vector<double> h_projw(nproj);
   host_projection_weights(p->M, p->m, h_projw.data());
   if (hipMemcpyAsync(d.ktab, ktab.data(), ktab.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) !=
@@ -404,11 +528,16 @@ int teuk_create(const teuk_params *p, const double *r_c, const double *theta, te
       hipMemcpyAsync(projw, h_projw.data(), h_projw.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) !=
           hipSuccess)
     return fail(TEUK_EHIP);
-  if (launch_init_coeffs(g, d, s->stream) != hipSuccess) return fail(TEUK_EHIP);
+  if ((g.stag ? launch_init_so2(g, d, s->stream) : launch_init_coeffs(g, d, s->stream)) != hipSuccess)
+    return fail(TEUK_EHIP);
   if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(TEUK_EHIP);
   *out = s;
   return TEUK_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 void teuk_destroy(teuk_solver *s) {
   if (!s) return;
@@ -482,9 +611,10 @@ int teuk_set_source(teuk_solver *s, const teuk_source_step *steps, int nsteps) {
 int teuk_set_trajectory(teuk_solver *s, const double *rows, int nrows, int nsteps) {
   if (!s || !rows || nrows <= 0 || nsteps <= 0) return TEUK_EINVAL;
   std::vector<SrcStep> sched;
-  int rc = consume_trajectory(rows, nrows, nsteps, s->g.start, s->P.dt, sched);
+  int rc = s->g.stag ? consume_trajectory_half(rows, nrows, nsteps, s->g.start, s->P.dt, sched)
+                     : consume_trajectory(rows, nrows, nsteps, s->g.start, s->P.dt, sched);
   if (rc) return rc;
-  return teuk_set_source(s, reinterpret_cast<const teuk_source_step *>(sched.data()), nsteps);
+  return teuk_set_source(s, reinterpret_cast<const teuk_source_step *>(sched.data()), (int)sched.size());
 }
 
 int teuk_trajectory_schedule(const double *rows, int nrows, int nsteps, double start, double dt,
@@ -497,11 +627,21 @@ int teuk_trajectory_schedule(const double *rows, int nrows, int nsteps, double s
   return TEUK_OK;
 }
 
+int teuk_trajectory_schedule_staggered(const double *rows, int nrows, int nsteps, double start, double dt,
+                                       teuk_source_step *out) {
+  if (!rows || !out || nrows <= 0 || nsteps <= 0 || !(dt > 0.0)) return TEUK_EINVAL;
+  std::vector<SrcStep> sched;
+  int rc = consume_trajectory_half(rows, nrows, nsteps, start, dt, sched);
+  if (rc) return rc;
+  std::memcpy(out, sched.data(), sched.size() * sizeof(SrcStep));
+  return TEUK_OK;
+}
+
 int teuk_step(teuk_solver *s, int nsteps) {
   if (!s || nsteps < 0) return TEUK_EINVAL;
   if (s->batch) return TEUK_EINVAL;  // a batch drives this solver
   if (s->k_host + nsteps > s->nt) return TEUK_ERANGE;
-  if (s->has_src && s->k_host + nsteps > s->d.nsched) return TEUK_ERANGE;
+  if (s->has_src && s->k_host + nsteps > sched_steps(s)) return TEUK_ERANGE;
   int rc = TEUK_OK;
   // Direct launches of n steps, closed by one finalize.
   auto direct = [&](int n) {
@@ -609,6 +749,10 @@ int teuk_body(const teuk_params *p, const double *r_c, const double *theta, cons
   teuk_solver *s = nullptr;
   if ((rc = teuk_create(p, r_c, theta, &s))) return rc;
   const int nt = s->nt;
+  const bool stag = p->integrator == 2;
+  // Staggered scheme: the particle radius after each step's second read,
+  // which gates the output rows (Old/body.cu:166-169, :236).
+  std::vector<double> rp_step;
   if (traj_path) {
     std::vector<double> rows;
     if ((rc = read_traj_file(traj_path, rows))) {
@@ -619,7 +763,18 @@ int teuk_body(const teuk_params *p, const double *r_c, const double *theta, cons
       teuk_destroy(s);
       return rc;
     }
+    if (stag) {
+      std::vector<SrcStep> sch;
+      if ((rc = consume_trajectory_half(rows.data(), (int)(rows.size() / 10), nt, s->g.start, p->dt, sch))) {
+        teuk_destroy(s);
+        return rc;
+      }
+      rp_step.resize(nt);
+      for (int k = 0; k < nt; k++) rp_step[k] = sch[2 * (size_t)k + 1].rp;
+    }
   }
+  // body.cu:208 writes 7 (h+, hx) pairs, Old/body.cu:237 five.
+  const int nh = stag ? 10 : 14;
   FILE *fo = out_path ? std::fopen(out_path, "w") : nullptr;
   FILE *ff = full_path ? std::fopen(full_path, "w") : nullptr;
   if ((out_path && !fo) || (full_path && !ff)) {
@@ -651,16 +806,17 @@ int teuk_body(const teuk_params *p, const double *r_c, const double *theta, cons
       }
       for (int i = 0; i < got; i++) {
         const teuk_output_row &o = rows[i];
-        if (o.write) {
+        const bool keep = o.write && (rp_step.empty() || rp_step[o.k] <= 12.0);
+        if (keep) {
           // body.cu:208 (time column is timer - Xmax)
           if (fo) {
             std::fprintf(fo, "%lf", o.t - p->xmax);
-            for (int j = 0; j < 14; j++) std::fprintf(fo, " %g", o.h[j]);
+            for (int j = 0; j < nh; j++) std::fprintf(fo, " %g", o.h[j]);
             std::fprintf(fo, "\n");
           }
           if (ff) {
             std::fprintf(ff, "%.17g", o.t - p->xmax);
-            for (int j = 0; j < 14; j++) std::fprintf(ff, " %.17g", o.h[j]);
+            for (int j = 0; j < nh; j++) std::fprintf(ff, " %.17g", o.h[j]);
             std::fprintf(ff, "\n");
           }
         }
@@ -680,7 +836,7 @@ int teuk_body(const teuk_params *p, const double *r_c, const double *theta, cons
 // ---------------------------------------------------------------------------
 // Launch shape of one solver's stage kernels.
 int teuk_set_tiling(teuk_solver *s, int waves_per_block, int waves_per_eu) {
-  if (!s || s->batch) return TEUK_EINVAL;
+  if (!s || s->batch || s->g.stag) return TEUK_EINVAL;
   if (!(waves_per_block == 1 || waves_per_block == 2 || waves_per_block == 4)) return TEUK_EINVAL;
   if (!(waves_per_eu == 2 || waves_per_eu == 4)) return TEUK_EINVAL;
   if (s->P.M % (kRows * waves_per_block) != 0) return TEUK_EINVAL;
@@ -698,7 +854,7 @@ int teuk_set_tiling(teuk_solver *s, int waves_per_block, int waves_per_eu) {
 // Batches.
 namespace {
 
-int member_steps(const teuk_solver *s) { return s->has_src ? std::min(s->nt, s->d.nsched) : s->nt; }
+int member_steps(const teuk_solver *s) { return s->has_src ? std::min(s->nt, sched_steps(s)) : s->nt; }
 
 void batch_drop_graph(teuk_batch *b) {
   if (b->gexec) {
@@ -805,7 +961,7 @@ int batch_build_graph(teuk_batch *b) {
 }  // namespace
 
 int teuk_batch_create(int integrator, long long hbm_budget, teuk_batch **out) {
-  if (!out || (integrator != 0 && integrator != 1) || hbm_budget < 0) return TEUK_EINVAL;
+  if (!out || (integrator != 0 && integrator != 1) || hbm_budget < 0) return TEUK_EINVAL;  // no staggered batches
   *out = nullptr;
   teuk_batch *b = new teuk_batch();
   b->integrator = integrator;

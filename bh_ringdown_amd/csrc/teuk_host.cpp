@@ -10,26 +10,21 @@
 
 namespace teuk {
 
-// TD_GPU_code/HighOrder/main.c:79-198.  rho_l = Xmin + l dx, theta_k = k dtheta,
-// Omega(rho) = 1 inside rin = Xmin + 0.64 (Xmax - Xmin) and
-// 1 - ((rho - rin)/(Xmax + 1e-6 - rin))^4 beyond, r* = rho/Omega, then
-// Newton on r*(r) - r* = 0 with dr*/dr = (r^2+a^2)/Delta, marching inward
-// from the outer edge with each point warm-started from its neighbour.
-void host_grid(int N, int M, double xmin, double xmax, double mass, double a, double *x_c, double *r_c,
-               double *theta) {
-  const double dx = (xmax - xmin) / N;
-  const double dth = kPi15 / M;
-  for (int l = 0; l < N; l++) x_c[l] = xmin + (double)l * dx;
-  for (int k = 0; k < M; k++) theta[k] = (double)k * dth;
+// TD_GPU_code/HighOrder/main.c:79-198 from the rho grid on: Omega(rho) = 1
+// inside rin = Xmin + 0.64 (Xmax - Xmin) and 1 - ((rho - rin)/(Xmax + 1e-6 -
+// rin))^4 beyond, r* = rho/Omega, then Newton on r*(r) - r* = 0 with
+// dr*/dr = (r^2+a^2)/Delta, marching inward from the outer edge with each
+// point warm-started from its neighbour.
+static void host_radii(int N, const double *x, double xmin, double xmax, double mass, double a, double *r_out) {
   const double rin = xmin + (xmax - xmin) * 0.64;
   const double outer = xmax + 1e-6;
   std::vector<double> target(N);
   for (int l = 0; l < N; l++) {
-    const double om = (x_c[l] < rin) ? 1.0 : 1.0 - std::pow((x_c[l] - rin) / (outer - rin), 4);
-    target[l] = x_c[l] / om;
+    const double om = (x[l] < rin) ? 1.0 : 1.0 - std::pow((x[l] - rin) / (outer - rin), 4);
+    target[l] = x[l] / om;
   }
   if (mass == 0.0) {
-    for (int l = 0; l < N; l++) r_c[l] = target[l];
+    for (int l = 0; l < N; l++) r_out[l] = target[l];
     return;
   }
   const double disc = std::sqrt(mass * mass - a * a);
@@ -60,8 +55,27 @@ void host_grid(int N, int M, double xmin, double xmax, double mass, double a, do
       step_rel = std::fabs(r_new - r) / r;
       r = r_new;
     } while (step_rel > tol);
-    r_c[l] = r_new;
+    r_out[l] = r_new;
   }
+}
+
+// main.c:79-198: rho_l = Xmin + l dx, theta_k = k dtheta and the radii.
+void host_grid(int N, int M, double xmin, double xmax, double mass, double a, double *x_c, double *r_c,
+               double *theta) {
+  const double dx = (xmax - xmin) / N;
+  const double dth = kPi15 / M;
+  for (int l = 0; l < N; l++) x_c[l] = xmin + (double)l * dx;
+  for (int k = 0; k < M; k++) theta[k] = (double)k * dth;
+  host_radii(N, x_c, xmin, xmax, mass, a, r_c);
+}
+
+// The staggered scheme's half points (TD_GPU_code/Old/kern.cu:93 places
+// its half-point coefficients at Xmin + (j - 1/2) dx with radius r_h[j]; Old/
+// has no main.c, so r_h is the same inversion at those points).
+void host_grid_half(int N, double xmin, double xmax, double mass, double a, double *x_h, double *r_h) {
+  const double dx = (xmax - xmin) / N;
+  for (int l = 0; l < N; l++) x_h[l] = xmin + ((double)l - 0.5) * dx;
+  host_radii(N, x_h, xmin, xmax, mass, a, r_h);
 }
 
 static double factorial(double n) {
@@ -119,12 +133,13 @@ void host_projection_weights(int M, double m, double *w) {
 // Evaluated with the host libm so that the device assembly reproduces the
 // host computation bit for bit.
 void host_coeff_tables(int N, int M, double xmin, double xmax, double dx, double mass, double a,
-                       double m, const double *r_c, const double *theta, double *ktab, double *rtab) {
+                       double m, const double *r, const double *theta, double *ktab, double *rtab, double shift,
+                       bool staggered) {
   const double rin = xmin + (xmax - xmin) * 0.64;
   const double outer = xmax + 1e-6;
   const double spin = a, em = m;
   for (int j = 0; j < N; j++) {
-    const double x = xmin + (double)j * dx;
+    const double x = xmin + ((double)j + shift) * dx;
     double om = 1.0, om1 = 0.0, om2 = 0.0;
     if (!(x < rin)) {
       om = 1.0 - std::pow((x - rin) / (outer - rin), 4);
@@ -132,7 +147,7 @@ void host_coeff_tables(int N, int M, double xmin, double xmax, double dx, double
       om2 = -12.0 * std::pow(x - rin, 2) / std::pow(outer - rin, 4);
     }
     const double L = om - x * om1;  // Omega^2 d(r*)/d(rho)
-    const double R = om * r_c[j];   // eR of kern.cu:130
+    const double R = om * r[j];     // eR of kern.cu:130
     const double a2 = spin * spin, a4 = a2 * a2;
     const double om_2 = om * om, om3 = om_2 * om, om4 = om_2 * om_2, om5 = om4 * om;
     const double S = R * R + om_2 * a2;
@@ -173,7 +188,7 @@ void host_coeff_tables(int N, int M, double xmin, double xmax, double dx, double
     rtab[k] = sn * sn;
     rtab[M + k] = cs;
     rtab[2 * M + k] = 4.0 * (cot * cot) - 4.0 * em * cot * csc + (em * em) * (csc * csc);
-    rtab[3 * M + k] = std::tan(0.5 * kPi15 - y);  // kern.cu:204
+    rtab[3 * M + k] = staggered ? cs / sn : std::tan(0.5 * kPi15 - y);  // Old/kern.cu:241-243, kern.cu:204
   }
 }
 

@@ -60,6 +60,8 @@ struct Grid {
   double xmin, xmax, dx, dtheta, dt;
   double mass, a, m;
   double start;
+  int stag;            // 1: staggered second-order scheme (TD_GPU_code/Old);
+                       // its source schedule holds two entries per step
 };
 
 // Device control block, one per solver.
@@ -112,6 +114,11 @@ struct DevState {
   double *scratch;         // [M][N] set-up scratch (bb plane)
   double *theta, *r_c;     // [M], [N]
   double *phase;           // [2][N] cos, sin of the source phase (kern.cu:2671-2672)
+  // Staggered scheme only: the half-point set (rho_a - dx/2, r_h).
+  double *ktab_h;          // [kNK][N] column constants at the half points
+  double *dbdx_h;          // [M][N] rho derivative of bb at the half points
+  double *r_h;             // [N]
+  double *phase_h;         // [2][N]
   double *patch;           // [kRing][2][ptsy][ptsx] source patches
   int *patch_org;          // [kRing][2] {row0, col0}
   SrcDev *sched;           // [nsched] source schedule, or null
@@ -149,6 +156,11 @@ hipError_t launch_source_at(const Grid &g, const DevState &d, int kstep, hipStre
 hipError_t launch_stage(const Grid &g, const DevState &d, int integrator, int stage,
                         bool has_src, int k_rel, Tiling t, hipStream_t s);
 hipError_t launch_finalize(const Grid &g, const DevState &d, int n, hipStream_t s);
+// Staggered second-order scheme (teuk_so2.hip): bb and dbdx of both point
+// sets, and one step (half step, its theta edges, full step, edges and
+// extraction).
+hipError_t launch_init_so2(const Grid &g, const DevState &d, hipStream_t s);
+hipError_t launch_so2_step(const Grid &g, const DevState &d, bool has_src, int k_rel, hipStream_t s);
 // Batched forms: one launch covers every member of the batch.
 hipError_t launch_source_batch(const SolveDesc *descs, int nsolves, int max_cells, int k_rel0, int n,
                                hipStream_t s);
@@ -162,12 +174,16 @@ bool source_available();
 // Host helpers (teuk_host.cpp).
 void host_grid(int N, int M, double xmin, double xmax, double mass, double a,
                double *x_c, double *r_c, double *theta);
+// Half points of the staggered scheme: rho_h = xmin + (l - 1/2) dx, r_h(rho_h).
+void host_grid_half(int N, double xmin, double xmax, double mass, double a, double *x_h, double *r_h);
 double host_swsh(double ell, double ml, double the);
 void host_projection_weights(int M, double m, double *w /*[7][5*(M/4)][3]*/);
-// Column constants [kNK][N] and row values [kNR][M] of the coefficient fields.
+// Column constants [kNK][N] at rho = xmin + (j + shift) dx with radius r[j]
+// and row values [kNR][M] of the coefficient fields; `staggered` selects the
+// staggered scheme's cot (cos/sin, Old/kern.cu:241-243) for the last row value.
 void host_coeff_tables(int N, int M, double xmin, double xmax, double dx, double mass, double a,
-                       double m, const double *r_c, const double *theta, double *ktab,
-                       double *rtab);
+                       double m, const double *r, const double *theta, double *ktab,
+                       double *rtab, double shift = 0.0, bool staggered = false);
 // Source phase e^{-i m a ln((r-r+)/(r-r-))/(r+-r-)} per column, [2][N].
 void host_source_phase(int N, double mass, double a, double m, const double *r_c, double *phase);
 // Per-step transcendentals and patch placement of the source (kern.cu:1186-1206, 2673).

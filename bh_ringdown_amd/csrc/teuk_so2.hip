@@ -1,0 +1,293 @@
+// HIP kernels of the staggered second-order scheme of TD_GPU_code/Old
+// (integrator 2) on gfx950.
+//
+// The reference runs nine kernels and a host read-back per step
+// (Old/body.cu:121-243): average to the half points, source, rhs, half
+// update, theta edges; average back, source, rhs, full update, theta edges,
+// rho edges.  Here one step is four launches:
+//   k_so2_half     averages, right-hand side and half update at the half
+//                  points rho_{a-1/2}, straight from the grid-point fields
+//                  (Old/kern.cu:196-303);
+//   k_so2_half_bc  the half-point fields' theta edge rows (:305-340);
+//   k_so2_full     re-averages, right-hand side and full update at the grid
+//                  points, in place (:346-453);
+//   k_so2_bc       theta edge rows, then rho edge columns, then the time
+//                  integrals of Psi at the extraction column (:456-522 and
+//                  Old/body.cu:199-216).
+// Each thread owns one point; a wavefront covers 64 consecutive rho columns
+// of one theta row, so every access is coalesced; the neighbour reads (one
+// column to the side, one row up and down) are cache hits.  The coefficient
+// fields are rebuilt per point from the column constants of their point set
+// and the row values, as in the high-order stage kernel.
+//
+// All arithmetic goes through teuk_point.h, built without contraction, so the
+// results equal the oracle's bit for bit.
+#include "teuk_internal.h"
+
+namespace teuk {
+namespace {
+
+using teuk_pt::Coef;
+
+constexpr int kSo2Rows = 4;   // theta rows per block (64 x 4 threads)
+
+__device__ __forceinline__ Coef so2_coef(const double *__restrict__ ktab, int N, int a, double sn2, double cs,
+                                         double Y) {
+  double K[kNK];
+#pragma unroll
+  for (int i = 0; i < kNK; i++) K[i] = ktab[(size_t)i * N + a];
+  return teuk_pt::coef_point(K, sn2, cs, Y, [](double x) { return __dsqrt_rn(x); });
+}
+
+// bb on rows 1 <= k < M-1 (Old/kern.cu:90 and :143, bb at :123 and :176) of
+// one point set into a scratch plane, then its second-order rho derivative
+// on 1 <= a < N-1 (Old/kern.cu:231-233, :381-383), zero elsewhere.
+__global__ __launch_bounds__(256) void k_so2_init_bb(Grid g, const double *__restrict__ ktab,
+                                                     const double *__restrict__ rtab, double *__restrict__ bbp) {
+  const size_t NM = (size_t)g.N * g.M;
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= NM) return;
+  const int k = (int)(id / g.N), j = (int)(id - (size_t)k * g.N);
+  double bb = 0.0;
+  if (k >= 1 && k < g.M - 1) bb = so2_coef(ktab, g.N, j, rtab[k], rtab[g.M + k], rtab[2 * g.M + k]).bb;
+  bbp[id] = bb;
+}
+
+__global__ __launch_bounds__(256) void k_so2_init_dbdx(Grid g, const double *__restrict__ bbp,
+                                                       double *__restrict__ dbdx) {
+  const size_t NM = (size_t)g.N * g.M;
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= NM) return;
+  const int a = (int)(id % g.N);
+  double v = 0.0;
+  if (a >= 1 && a < g.N - 1) v = teuk_pt::so2_dbdx(bbp[id + 1], bbp[id - 1], g.dx);
+  dbdx[id] = v;
+}
+
+// The source value of point (b, a) from the patch of half step kh, or zero.
+__device__ __forceinline__ void so2_source(const Grid &g, const DevState &d, int kh, int b, int a, double &sr,
+                                           double &si) {
+  const int slot = kh % kRing;
+  const int it0 = d.patch_org[2 * slot], ip0 = d.patch_org[2 * slot + 1];
+  const int sb = b - it0, sa = a - ip0;
+  sr = 0.0;
+  si = 0.0;
+  if ((unsigned)sb < (unsigned)g.ptsy && (unsigned)sa < (unsigned)g.ptsx) {
+    const double *patch = d.patch + (size_t)slot * 2 * g.ptsx * g.ptsy;
+    const int pc = sb * g.ptsx + sa;
+    sr = patch[pc];
+    si = patch[g.ptsx * g.ptsy + pc];
+  }
+}
+
+// Right-hand side at one point of either set (Old/kern.cu:214-284,
+// :365-434): pv = the averaged fields at rows b-1, b, b+1 (q) and b (p),
+// dxv = the four rho differences.
+__device__ __forceinline__ void so2_rhs_point(const Grid &g, const DevState &d, const double *__restrict__ ktab,
+                                              const double *__restrict__ dbdx, int b, int a, const double qr[3],
+                                              const double qi[3], double pr, double pi, const double dxv[4],
+                                              double sr, double si, double r[4]) {
+  const double *__restrict__ rtab = d.rtab;
+  const int M = g.M;
+  const Coef cf = so2_coef(ktab, g.N, a, rtab[b], rtab[M + b], rtab[2 * M + b]);
+  const double dre = teuk_pt::fma_(-dbdx[(size_t)b * g.N + a], cf.ee, cf.dre);
+  const double ct = rtab[3 * M + b];   // cos/sin (Old/kern.cu:241-243)
+  teuk_pt::Pt x;
+  x.qr = qr[1];
+  x.qi = qi[1];
+  x.pr = pr;
+  x.pi = pi;
+  x.llr = teuk_pt::fma_(ct, teuk_pt::so2_dth(qr[2], qr[0], g.dtheta), teuk_pt::so2_dth2(qr[1], qr[2], qr[0], g.dtheta));
+  x.lli = teuk_pt::fma_(ct, teuk_pt::so2_dth(qi[2], qi[0], g.dtheta), teuk_pt::so2_dth2(qi[1], qi[2], qi[0], g.dtheta));
+  x.qrx = dxv[0];
+  x.qix = dxv[1];
+  x.prx = dxv[2];
+  x.pix = dxv[3];
+  x.kqr = x.kqi = x.kpr = x.kpi = 0.0;
+  x.sr = sr;
+  x.si = si;
+  teuk_pt::rhs(cf, dre, x, r);
+}
+
+// Half step: X_h = avg(X[a], X[a-1]) + dt/2 L_h, rows 1..M-2, 1 <= a < N
+// (Old/kern.cu:196-303).  Reads X (d.U) only, writes X_h (d.C).
+__global__ __launch_bounds__(256) void k_so2_half(Grid g, DevState d, int has_src, int k_rel) {
+  const int a = blockIdx.x * kWave + threadIdx.x;
+  const int b = blockIdx.y * kSo2Rows + threadIdx.y;
+  const int N = g.N, M = g.M;
+  if (a < 1 || a >= N || b < 1 || b >= M - 1) return;
+  const size_t NM = (size_t)N * M;
+  const double *__restrict__ X = d.U;
+  double *__restrict__ Xh = d.C;
+  const size_t c = (size_t)b * N + a;
+  double qr[3], qi[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const size_t ci = c + (size_t)(i - 1) * N;
+    qr[i] = teuk_pt::so2_avg(X[ci], X[ci - 1]);
+    qi[i] = teuk_pt::so2_avg(X[NM + ci], X[NM + ci - 1]);
+  }
+  const double pr = teuk_pt::so2_avg(X[2 * NM + c], X[2 * NM + c - 1]);
+  const double pi = teuk_pt::so2_avg(X[3 * NM + c], X[3 * NM + c - 1]);
+  double dxv[4];
+#pragma unroll
+  for (int f = 0; f < 4; f++) dxv[f] = teuk_pt::so2_dx(X[f * NM + c], X[f * NM + c - 1], g.dx);
+  double sr = 0.0, si = 0.0;
+  if (has_src) so2_source(g, d, 2 * (d.ctl->k + k_rel), b, a, sr, si);
+  double r[4];
+  so2_rhs_point(g, d, d.ktab_h, d.dbdx_h, b, a, qr, qi, pr, pi, dxv, sr, si, r);
+  const double h = 0.5 * g.dt;
+  const double v0[4] = {qr[1], qi[1], pr, pi};
+#pragma unroll
+  for (int f = 0; f < 4; f++) Xh[f * NM + c] = teuk_pt::axpy(h, r[f], v0[f]);
+}
+
+// Theta edge rows of the half-point fields, every column (Old/kern.cu:305-340).
+// Column 0 of X_h is never written by the half step and stays zero.
+__global__ __launch_bounds__(256) void k_so2_half_bc(Grid g, DevState d) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  const int N = g.N, M = g.M;
+  if (a >= N) return;
+  const size_t NM = (size_t)N * M;
+  double *__restrict__ Xh = d.C;
+#pragma unroll
+  for (int f = 0; f < 4; f++) {
+    double *p = Xh + f * NM + a;
+    const double lo = g.even_m ? teuk_pt::so2_edge(p[(size_t)N], p[2 * (size_t)N]) : 0.0;
+    const double hi = g.even_m ? teuk_pt::so2_edge(p[(size_t)(M - 2) * N], p[(size_t)(M - 3) * N]) : 0.0;
+    p[0] = lo;
+    p[(size_t)(M - 1) * N] = hi;
+  }
+}
+
+// Full step: X += dt L_c with X_c = avg(X_h[a], X_h[a+1]), rows 1..M-2,
+// 1 <= a < N-1 (Old/kern.cu:346-453).  Reads X_h, updates X in place (each
+// thread reads only its own point of X).
+__global__ __launch_bounds__(256) void k_so2_full(Grid g, DevState d, int has_src, int k_rel) {
+  const int a = blockIdx.x * kWave + threadIdx.x;
+  const int b = blockIdx.y * kSo2Rows + threadIdx.y;
+  const int N = g.N, M = g.M;
+  if (a < 1 || a >= N - 1 || b < 1 || b >= M - 1) return;
+  const size_t NM = (size_t)N * M;
+  const double *__restrict__ Xh = d.C;
+  double *__restrict__ X = d.U;
+  const size_t c = (size_t)b * N + a;
+  double qr[3], qi[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const size_t ci = c + (size_t)(i - 1) * N;
+    qr[i] = teuk_pt::so2_avg(Xh[ci], Xh[ci + 1]);
+    qi[i] = teuk_pt::so2_avg(Xh[NM + ci], Xh[NM + ci + 1]);
+  }
+  const double pr = teuk_pt::so2_avg(Xh[2 * NM + c], Xh[2 * NM + c + 1]);
+  const double pi = teuk_pt::so2_avg(Xh[3 * NM + c], Xh[3 * NM + c + 1]);
+  double dxv[4];
+#pragma unroll
+  for (int f = 0; f < 4; f++) dxv[f] = teuk_pt::so2_dx(Xh[f * NM + c + 1], Xh[f * NM + c], g.dx);
+  double sr = 0.0, si = 0.0;
+  if (has_src) so2_source(g, d, 2 * (d.ctl->k + k_rel) + 1, b, a, sr, si);
+  double r[4];
+  so2_rhs_point(g, d, d.ktab, d.dbdx, b, a, qr, qi, pr, pi, dxv, sr, si, r);
+#pragma unroll
+  for (int f = 0; f < 4; f++) X[f * NM + c] = teuk_pt::axpy(g.dt, r[f], X[f * NM + c]);
+}
+
+// Edges and extraction.  Blocks 0 .. nb-2: theta edge rows of columns
+// 1 <= a <= N-2 (Old/kern.cu:456-497; its column N-1 value is overwritten by
+// the rho fill).  Last block, one thread per theta row b: the rho edge
+// columns of row b (:499-522), taking row 0 / M-1 values at columns 1, 2,
+// N-3, N-2 from the theta fill it recomputes itself (so no thread waits for
+// another), then the time integrals of Psi at the extraction column
+// (Old/body.cu:199-216) and the Fr-step snapshot.
+__global__ __launch_bounds__(256) void k_so2_bc(Grid g, DevState d, int k_rel) {
+  const int N = g.N, M = g.M;
+  const size_t NM = (size_t)N * M;
+  double *__restrict__ X = d.U;
+  const int t = threadIdx.x;
+  if (blockIdx.x + 1 < gridDim.x) {
+    const int a = 1 + blockIdx.x * blockDim.x + t;
+    if (a > N - 2) return;
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+      double *p = X + f * NM + a;
+      const double lo = g.even_m ? teuk_pt::so2_edge(p[(size_t)N], p[2 * (size_t)N]) : 0.0;
+      const double hi = g.even_m ? teuk_pt::so2_edge(p[(size_t)(M - 2) * N], p[(size_t)(M - 3) * N]) : 0.0;
+      p[0] = lo;
+      p[(size_t)(M - 1) * N] = hi;
+    }
+    return;
+  }
+  const int b = t;
+  if (b >= M) return;
+  // Final value of (b, col) for 1 <= col <= N-2: the theta fill on the edge
+  // rows, the updated field elsewhere.
+  auto val = [&](int f, int col) -> double {
+    const double *p = X + f * NM + col;
+    if (b == 0) return g.even_m ? teuk_pt::so2_edge(p[(size_t)N], p[2 * (size_t)N]) : 0.0;
+    if (b == M - 1) return g.even_m ? teuk_pt::so2_edge(p[(size_t)(M - 2) * N], p[(size_t)(M - 3) * N]) : 0.0;
+    return p[(size_t)b * N];
+  };
+  double e0[4], e1[4], ej[4];
+  const int je = g.je;
+#pragma unroll
+  for (int f = 0; f < 4; f++) {
+    e0[f] = teuk_pt::so2_edge(val(f, 1), val(f, 2));
+    e1[f] = teuk_pt::so2_edge(val(f, N - 2), val(f, N - 3));
+    ej[f] = je == 0 ? e0[f] : (je == N - 1 ? e1[f] : val(f, je));
+  }
+#pragma unroll
+  for (int f = 0; f < 4; f++) {
+    X[f * NM + (size_t)b * N] = e0[f];
+    X[f * NM + (size_t)b * N + N - 1] = e1[f];
+  }
+  // Old/body.cu:209-216: rectangle-rule integrals, every step.
+  const Ext &ext = d.ext;
+  Ctl *ctl = d.ctl;
+  const int kstep = ctl->k + k_rel;
+  const double dt = g.dt;
+  const double q0 = ej[0], q1 = ej[1];
+  const double ir = ext.integ[b] + q0 * dt;
+  const double ii = ext.integ[M + b] + q1 * dt;
+  const double dr = ext.integ[2 * M + b] + ir * dt;
+  const double di = ext.integ[3 * M + b] + ii * dt;
+  ext.integ[b] = ir;
+  ext.integ[M + b] = ii;
+  ext.integ[2 * M + b] = dr;
+  ext.integ[3 * M + b] = di;
+  if (!isfinite(q0) || !isfinite(q1)) ctl->status = 1;
+  if ((kstep % g.Fr) == 0 && kstep / g.Fr < g.nrec_cap) {
+    double *sn = ext.snap + (size_t)(kstep / g.Fr) * 6 * M;
+    sn[b] = q0;
+    sn[M + b] = q1;
+    sn[2 * M + b] = ir;
+    sn[3 * M + b] = ii;
+    sn[4 * M + b] = dr;
+    sn[5 * M + b] = di;
+  }
+}
+
+}  // namespace
+
+hipError_t launch_init_so2(const Grid &g, const DevState &d, hipStream_t s) {
+  const size_t NM = (size_t)g.N * g.M;
+  const dim3 nb((unsigned)((NM + 255) / 256));
+  hipLaunchKernelGGL(k_so2_init_bb, nb, dim3(256), 0, s, g, d.ktab_h, d.rtab, d.scratch);
+  hipLaunchKernelGGL(k_so2_init_dbdx, nb, dim3(256), 0, s, g, d.scratch, d.dbdx_h);
+  hipLaunchKernelGGL(k_so2_init_bb, nb, dim3(256), 0, s, g, d.ktab, d.rtab, d.scratch);
+  hipLaunchKernelGGL(k_so2_init_dbdx, nb, dim3(256), 0, s, g, d.scratch, d.dbdx);
+  return hipGetLastError();
+}
+
+hipError_t launch_so2_step(const Grid &g, const DevState &d, bool has_src, int k_rel, hipStream_t s) {
+  if (g.N % kWave != 0 || g.M % kSo2Rows != 0 || g.M > 256) return hipErrorInvalidValue;
+  const dim3 grid(g.N / kWave, g.M / kSo2Rows), block(kWave, kSo2Rows);
+  const int hs = has_src ? 1 : 0;
+  hipLaunchKernelGGL(k_so2_half, grid, block, 0, s, g, d, hs, k_rel);
+  hipLaunchKernelGGL(k_so2_half_bc, dim3((g.N + 255) / 256), dim3(256), 0, s, g, d);
+  hipLaunchKernelGGL(k_so2_full, grid, block, 0, s, g, d, hs, k_rel);
+  const int nb_theta = (g.N - 2 + 255) / 256;
+  hipLaunchKernelGGL(k_so2_bc, dim3(nb_theta + 1), dim3(256), 0, s, g, d, k_rel);
+  return hipGetLastError();
+}
+
+}  // namespace teuk

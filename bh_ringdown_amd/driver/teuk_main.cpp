@@ -13,7 +13,7 @@
 // main.c):
 //   N 16000   M 256   Xmax 50.0   Xmin -50.0   Fr 50   T 1500.0
 //   PTSX 128  PTSY 128   mass 1.0   aa 1e-8   dt 0.002
-//   mmin -5   mmax 5   integrator 0 (1 = RK4)   j_extract -1
+//   mmin -5   mmax 5   integrator 0 (1 = RK4, 2 = the staggered scheme of TD_GPU_code/Old)   j_extract -1
 // `name` selects <traj_dir>/<name>.traj (traj_dir defaults to
 // ../Trajectories as in body.cu:77); "-" runs without a particle source.
 #include <cmath>

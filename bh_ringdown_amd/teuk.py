@@ -67,7 +67,8 @@ EXPORTS = [
     "teuk_footprint", "teuk_body", "teuk_strerror", "teuk_trajectory_schedule", "teuk_traj_plunge",
     "teuk_set_tiling", "teuk_remaining", "teuk_batch_create", "teuk_batch_destroy", "teuk_batch_add",
     "teuk_batch_remove", "teuk_batch_set_tiling", "teuk_batch_step", "teuk_batch_sync", "teuk_batch_info",
-    "teuk_source_coeffs", "teuk_source_patch",
+    "teuk_source_coeffs", "teuk_source_patch", "teuk_grid_half", "teuk_create_staggered",
+    "teuk_trajectory_schedule_staggered",
 ]
 
 _lib = None
@@ -90,6 +91,10 @@ def lib():
                                  ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double],
                                 ctypes.c_int),
         "teuk_create": ([P, _dp, _dp, ctypes.POINTER(vp)], ctypes.c_int),
+        "teuk_create_staggered": ([P, _dp, _dp, _dp, ctypes.POINTER(vp)], ctypes.c_int),
+        "teuk_grid_half": ([P, _dp, _dp], ctypes.c_int),
+        "teuk_trajectory_schedule_staggered": ([_dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                ctypes.POINTER(SourceStep)], ctypes.c_int),
         "teuk_destroy": ([vp], None),
         "teuk_set_fields": ([vp, _dp], ctypes.c_int),
         "teuk_get_fields": ([vp, _dp], ctypes.c_int),
@@ -166,6 +171,22 @@ def grid(p):
     return x, r, th
 
 
+def grid_half(p):
+    """Half points of the staggered scheme (integrator 2): (x_h[N], r_h[N])."""
+    x = np.zeros(p.N); r = np.zeros(p.N)
+    _check(lib().teuk_grid_half(ctypes.byref(p), _ptr(x), _ptr(r)), "teuk_grid_half")
+    return x, r
+
+
+def trajectory_schedule_staggered(rows, nsteps, dt, start=0.0):
+    """The staggered scheme's source scalars, two per step (Old/body.cu:113-180)."""
+    rows = np.ascontiguousarray(rows, dtype=np.float64)
+    out = (SourceStep * (2 * nsteps))()
+    _check(lib().teuk_trajectory_schedule_staggered(_ptr(rows), rows.shape[0], nsteps, start, dt, out),
+           "teuk_trajectory_schedule_staggered")
+    return list(out)
+
+
 def trajectory_schedule(rows, nsteps, dt, start=0.0):
     """The per-step source scalars body.cu:97-129 derives from .traj rows."""
     rows = np.ascontiguousarray(rows, dtype=np.float64)
@@ -197,14 +218,19 @@ def footprint(p, nsteps=None):
 class Solver:
     """One m-mode solve resident on the GPU (a libteuk handle)."""
 
-    def __init__(self, p, r_c=None, theta=None):
+    def __init__(self, p, r_c=None, theta=None, r_h=None):
         self.p = p
         if r_c is None or theta is None:
             _, r_c, theta = grid(p)
         self.r_c, self.theta = np.ascontiguousarray(r_c), np.ascontiguousarray(theta)
         h = ctypes.c_void_p()
-        _check(lib().teuk_create(ctypes.byref(p), _ptr(self.r_c), _ptr(self.theta), ctypes.byref(h)),
-               "teuk_create")
+        if r_h is not None:   # staggered scheme with given half-point radii
+            self.r_h = np.ascontiguousarray(r_h, dtype=np.float64)
+            _check(lib().teuk_create_staggered(ctypes.byref(p), _ptr(self.r_h), _ptr(self.r_c), _ptr(self.theta),
+                                               ctypes.byref(h)), "teuk_create_staggered")
+        else:
+            _check(lib().teuk_create(ctypes.byref(p), _ptr(self.r_c), _ptr(self.theta), ctypes.byref(h)),
+                   "teuk_create")
         self._h = h
 
     def close(self):

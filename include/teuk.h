@@ -41,7 +41,9 @@ typedef struct teuk_params {
   double T;             /* stop time, parm.h:8                               */
   int    Fr;            /* output cadence in steps, parm.h:7                 */
   int    ptsx, ptsy;    /* source patch size, parm.h:10-11                   */
-  int    integrator;    /* 0 = SSP-RK3 (reference), 1 = classic RK4           */
+  int    integrator;    /* 0 = SSP-RK3 (reference), 1 = classic RK4,
+                           2 = the staggered second-order scheme of
+                           TD_GPU_code/Old (kern.cu, body.cu there)         */
   int    j_extract;     /* extraction column; -1 = N-1 (scri, kern.cu:750)   */
 } teuk_params;
 
@@ -73,6 +75,12 @@ typedef struct teuk_solver teuk_solver;
  * inversion of r*(r) = x/Omega(x)), theta[M].  Replaces grid(). */
 int teuk_grid(const teuk_params *p, double *x_c, double *r_c, double *theta);
 
+/* Half points of the staggered scheme (integrator 2): x_h[l] = xmin +
+ * (l - 1/2) dx, where TD_GPU_code/Old/kern.cu:93 places its half-point
+ * coefficients, and r_h[N] by the same Newton inversion as teuk_grid.  Old/
+ * has no main.c; this supplies the r_h its body() receives. */
+int teuk_grid_half(const teuk_params *p, double *x_h, double *r_h);
+
 /* Convenience: fill a teuk_params with the reference's derived quantities
  * (dx = (xmax-xmin)/N, dtheta = Pi/M, main.c:59-60), dt = dt_over_dx*dx,
  * s = -2, Fr = 1, ptsx = ptsy = 128, j_extract = -1. */
@@ -85,6 +93,13 @@ int teuk_default_params(teuk_params *p, int N, int M, double xmin, double xmax,
  * N must be a multiple of 64 and >= 128; M a multiple of 8 and >= 16. */
 int teuk_create(const teuk_params *p, const double *r_c, const double *theta,
                 teuk_solver **out);
+
+/* The staggered scheme (integrator 2) with caller-supplied half-point radii,
+ * as TD_GPU_code/Old/body.cu:28-30 receives r_h beside r_c and theta (and
+ * kernel_init computes both coefficient sets, Old/kern.cu:61-193).
+ * teuk_create with integrator 2 computes r_h by teuk_grid_half. */
+int teuk_create_staggered(const teuk_params *p, const double *r_h, const double *r_c,
+                          const double *theta, teuk_solver **out);
 void teuk_destroy(teuk_solver *s);
 
 /* Host <-> device field transfer, layout 4 x N x M (see top). */
@@ -92,7 +107,9 @@ int teuk_set_fields(teuk_solver *s, const double *u);
 int teuk_get_fields(teuk_solver *s, double *u);
 
 /* Source schedule: one entry per step (NULL / 0 = no source).  Replaces the
- * per-step reads of body.cu:112-129. */
+ * per-step reads of body.cu:112-129.  For the staggered scheme two entries
+ * per step: the half step's (evaluated at the half points), then the full
+ * step's (Old/body.cu:127-180). */
 int teuk_set_source(teuk_solver *s, const teuk_source_step *steps, int nsteps);
 
 /* Builds the source schedule from raw .traj rows (10 columns, the format of
@@ -107,8 +124,17 @@ int teuk_set_trajectory(teuk_solver *s, const double *rows, int nrows, int nstep
 int teuk_trajectory_schedule(const double *rows, int nrows, int nsteps, double start, double dt,
                              teuk_source_step *out);
 
+/* Host-only: the staggered scheme's schedule, out[2 * nsteps]: after the
+ * same priming, one line per half step, each used as read, accelerations by
+ * backward difference over dt/2 ((v - v_prev) / dt * 2), the step's timer
+ * in both entries (Old/body.cu:113-180). */
+int teuk_trajectory_schedule_staggered(const double *rows, int nrows, int nsteps, double start, double dt,
+                                       teuk_source_step *out);
+
 /* Advance nsteps full time steps (SSP-RK3: three fused stage kernels per
- * step; RK4: four).  Asynchronous on the solver's stream; use teuk_sync. */
+ * step; RK4: four; staggered: a half-step and a full-step kernel, each
+ * followed by its edge fill).  Asynchronous on the solver's stream; use
+ * teuk_sync. */
 int teuk_step(teuk_solver *s, int nsteps);
 int teuk_sync(teuk_solver *s);
 
@@ -169,7 +195,7 @@ int teuk_remaining(teuk_solver *s, int *steps);
  * work on it. */
 typedef struct teuk_batch teuk_batch;
 /* hbm_budget: bytes of member footprint (teuk_footprint) the batch admits;
- * 0 = no limit. */
+ * 0 = no limit.  Integrators 0 and 1 only. */
 int teuk_batch_create(int integrator, long long hbm_budget, teuk_batch **out);
 void teuk_batch_destroy(teuk_batch *b);      /* members stay valid */
 /* TEUK_ENOMEM when the member would exceed the budget. */
@@ -188,7 +214,10 @@ int teuk_batch_info(teuk_batch *b, int *members, int *active, long long *bytes_u
  * (traj_path, may be NULL for no source), evolves from zero fields to T,
  * writes the waveform file in the reference format "%lf %g x14" to out_path
  * (NULL: no file) and, if full_path is non-NULL, the same rows at %.17g.
- * Prints the reference's stdout lines when verbose. */
+ * Prints the reference's stdout lines when verbose.  With integrator 2 it
+ * is Old/body.cu:28-258 instead: r_h from teuk_grid_half, one trajectory
+ * line per half step, rows "%lf %g x10" (l = |m| .. |m|+4) written only
+ * while the particle radius is <= 12 (Old/body.cu:233-237). */
 int teuk_body(const teuk_params *p, const double *r_c, const double *theta,
               const char *traj_path, const char *out_path, const char *full_path,
               int verbose);

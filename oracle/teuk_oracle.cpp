@@ -138,7 +138,7 @@ namespace {
  * bir = K2 sin^2 - K1; the algebra is that of kern.cu:132-145 term by term,
  * only regrouped.  Column j sits at rho = Xmin + (j + shift) dx with radius
  * r[j] (shift 0: the grid points; -1/2: the half points of the staggered
- * scheme, Old/kern.cu:82-83). */
+ * scheme, Old/kern.cu:93). */
 enum { K1, K2, KA, KR, KD, KBt, KBr, KC0, KI0, KI1, KBi, KCi, NK };   /* teuk_point.h order */
 
 void column_constants(const orc_params *p, double shift, const double *r_col, std::vector<double> &K) {

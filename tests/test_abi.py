@@ -49,7 +49,7 @@ def test_host_grid_equals_oracle(N, M, xmin, xmax, a):
 
 def test_create_rejects_bad_arguments_before_touching_the_gpu():
     bad = [
-        dict(N=100), dict(M=12), dict(M=20), dict(s=-1.0), dict(Fr=0), dict(integrator=2),
+        dict(N=100), dict(M=12), dict(M=20), dict(s=-1.0), dict(Fr=0), dict(integrator=3),
         dict(j_extract=5000), dict(a=1.0), dict(m=1.5), dict(dt=0.0),
     ]
     for kw in bad:
@@ -113,6 +113,48 @@ def test_trajectory_consumption_pattern():
     with pytest.raises(teuk.TeukError) as e:
         teuk.trajectory_schedule(rows, 30, dt)      # needs 2 + 60 rows
     assert e.value.code == teuk.TEUK_ERANGE
+
+
+def test_staggered_trajectory_consumption_pattern():
+    """Old/body.cu:113-180: the same priming, then one row per half step, each
+    used as read; accelerations over the half step; the step's timer in
+    both entries."""
+    dt = 0.1
+    rows = synthetic_traj(50, dt)
+    sched = teuk.trajectory_schedule_staggered(rows, 10, dt)
+    assert len(sched) == 20
+    vprev = np.zeros(3)
+    for i, st in enumerate(sched):
+        row = rows[i + 2]          # rows 0 and 1 prime; half step i uses row i + 2
+        assert st.rp == row[1] + 1e-6
+        assert (st.tp, st.drdt, st.dthdt, st.dphidt, st.phip, st.E, st.lz, st.Q) == \
+            (row[2], row[3], row[4], row[5], row[6], row[7], row[8], row[9])
+        assert st.d2rdt2 == (row[3] - vprev[0]) / dt * 2.0
+        assert st.d2phidt2 == (row[5] - vprev[2]) / dt * 2.0
+        assert st.timer == (i // 2) * dt or abs(st.timer - (i // 2) * dt) < 1e-15
+        vprev = row[3:6]
+    with pytest.raises(teuk.TeukError) as e:
+        teuk.trajectory_schedule_staggered(rows, 30, dt)
+    assert e.value.code == teuk.TEUK_ERANGE
+
+
+@pytest.mark.parametrize("N,xmin,xmax,a", [(1024, -100.0, 300.0, 0.5), (4096, -600.0, 1600.0, 0.99)])
+def test_host_half_grid_equals_oracle(N, xmin, xmax, a):
+    p = teuk.default_params(N, 32, xmin, xmax, a, 2)
+    x, r = teuk.grid_half(p)
+    xo, ro = O.grid_half(N, xmin, xmax, 1.0, a)
+    assert np.array_equal(x, xo) and np.array_equal(r, ro)
+
+
+def test_staggered_footprint():
+    """Two field states instead of SSP-RK3's three, plus the half points'
+    dbdx plane, column constants, radii and source phases."""
+    N, M = 4096, 32
+    p = teuk.default_params(N, M, -400.0, 1200.0, 0.5, 2, T=100.0)
+    f0 = teuk.footprint(p, 0)
+    p.integrator = 2
+    f2 = teuk.footprint(p, 0)
+    assert f0 - f2 == 8 * (4 * N * M - (N * M + (12 + 3) * N))
 
 
 def test_driver_rejects_unknown_parameter(tmp_path):

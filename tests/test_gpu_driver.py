@@ -69,3 +69,49 @@ def test_driver_writes_reference_format(tmp_path):
         assert len(lines) == len(want)
         for (t, h), line in zip(want, lines):
             assert line == "%f" % t + "".join(" %g" % v for v in h)
+
+
+def test_driver_staggered_scheme(tmp_path):
+    """integrator 2 (TD_GPU_code/Old): rows "%lf %g x10" (l = |m| .. |m|+4)
+    written when k % Fr == 0, k dt >= Xmax + 200 and the particle radius of
+    the step's second read is <= 12 (Old/body.cu:233-237), against the
+    oracle's staggered run."""
+    assert os.path.exists(DRIVER), "driver not built (bh_ringdown_amd.build)"
+    N, M, xmin, xmax, a, m = 256, 16, -40.0, 60.0, 0.6, 2
+    dt = 0.5 * (xmax - xmin) / N
+    T, Fr = 290.0, 10
+    nt = int(T / dt)
+    rows = teuk.plunge_trajectory(a, 1e-2, 0.3, dt / 2, (2 * nt + 6) * dt / 2, seed=5, r0_over_isco=1.3)
+    (tmp_path / "Trajectories").mkdir()
+    teuk.write_traj(tmp_path / "Trajectories" / "plunge.traj", rows)
+    rows_back = np.loadtxt(tmp_path / "Trajectories" / "plunge.traj")
+    params = tmp_path / "params.txt"
+    params.write_text("\n".join([f"N {N}", f"M {M}", f"Xmin {xmin!r}", f"Xmax {xmax!r}", f"aa {a!r}",
+                                 f"dt {dt!r}", f"T {T!r}", f"Fr {Fr}", "PTSX 128", "PTSY 128", f"mmin {m}",
+                                 f"mmax {m}", "integrator 2", "j_extract -1"]) + "\n")
+    print("driver: teuk_run, staggered scheme, 256x16", flush=True)
+    res = subprocess.run([DRIVER, "-p", str(params), "-t", str(tmp_path / "Trajectories"), "-o", str(tmp_path),
+                          "plunge"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    q = teuk.default_params(N, M, xmin, xmax, a, m, dt_over_dx=0.5)
+    q.dt = dt
+    _, r_c, th = teuk.grid(q)
+    _, r_h = teuk.grid_half(q)
+    sched = teuk.trajectory_schedule_staggered(rows_back, nt, dt)
+    ost = [O.SrcStep(*[getattr(x, f) for f, _ in teuk.SourceStep._fields_]) for x in sched]
+    op = O.make_params(N, M, xmin, xmax, a, m, dt=dt, integrator=2)
+    u = np.zeros((4, M, N))
+    rc, _, h, _ = O.run_so2(op, r_h, r_c, th, u, nt, src_steps=ost)
+    assert rc == 0
+    want, t = [], 0.0
+    for k in range(nt):
+        if k % Fr == 0 and k * dt >= xmax + 200.0 and sched[2 * k + 1].rp <= 12.0:
+            want.append((t - xmax, h[k][:10]))
+        t += dt
+    assert len(want) > 3
+    full = np.loadtxt(tmp_path / "hm02_plunge.full.dat")
+    assert full.shape == (len(want), 11)
+    for (tw, hw), row in zip(want, full):
+        assert row[0] == tw and np.array_equal(row[1:], hw)
+    lines = (tmp_path / "hm02_plunge.dat").read_text().splitlines()
+    assert [l for l in lines] == ["%f" % tw + "".join(" %g" % v for v in hw) for tw, hw in want]

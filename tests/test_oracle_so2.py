@@ -101,7 +101,7 @@ def test_edges_are_even_quadratic_extrapolation(edge):
 
 def test_theta_rows_before_rho_columns_and_odd_m_zero():
     """The corner takes the rho extrapolation of the theta-filled edge row
-    (Old/body.cu:176-181 runs boundary2a before boundary2b): separable data
+    (Old/body.cu:191-197 runs boundary2a before boundary2b): separable data
     g(b) h(a), each even quadratic about the corner, return g(0) h(0) there
     even when the old edge rows hold garbage.  Odd m zeroes the theta rows."""
     p = params(m=2)

@@ -29,9 +29,14 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 # Algorithmic bytes per point-update (state traffic of one sweep per stage):
-# SSP-RK3 64 + 96 + 96; classic RK4 in accumulator form 96 + 160 + 160 + 96.
-B_ALG = {"rk3": 256.0, "rk4": 512.0}
-STAGES = {"rk3": 3, "rk4": 4}
+# SSP-RK3 64 + 96 + 96; classic RK4 in accumulator form 96 + 160 + 160 + 96;
+# the staggered second-order scheme 64 (half step: fields in, half-point
+# fields out) + 96 (full step: half-point fields in, fields in and out).
+B_ALG = {"rk3": 256.0, "rk4": 512.0, "so2": 160.0}
+STAGES = {"rk3": 3, "rk4": 4, "so2": 2}
+INTEGRATORS = {"rk3": 0, "rk4": 1, "so2": 2}
+INTEGRATOR_NAMES = {"rk4": "classic RK4", "rk3": "SSP-RK3",
+                    "so2": "staggered second-order scheme (TD_GPU_code/Old)"}
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_stage_rk4_16384x64_plunge_pmc.txt")
 
 
@@ -68,14 +73,19 @@ def cpu_baseline(p, rows, seconds=12.0):
     _, r_c, th = teuk.grid(p)
     op = O.make_params(p.N, p.M, p.xmin, p.xmax, p.a, p.m, dt=p.dt, j_extract=p.j_extract, nthreads=nth,
                        integrator=p.integrator)
-    name = "RK4" if p.integrator == 1 else "SSP-RK3"
+    name = {0: "SSP-RK3", 1: "RK4", 2: "staggered-scheme"}[p.integrator]
+    stag = p.integrator == 2
+    r_h = teuk.grid_half(p)[1] if stag else None
 
     def timed(n):
-        sched = teuk.trajectory_schedule(rows, n, p.dt)
+        sched = (teuk.trajectory_schedule_staggered if stag else teuk.trajectory_schedule)(rows, n, p.dt)
         ost = [O.SrcStep(*[getattr(x, f) for f, _ in teuk.SourceStep._fields_]) for x in sched]
         u = np.zeros((4, p.M, p.N))
         t0 = time.perf_counter()
-        rc, *_ = O.run(op, r_c, th, u, n, src_steps=ost, timing=True)
+        if stag:
+            rc, *_ = O.run_so2(op, r_h, r_c, th, u, n, src_steps=ost, timing=True)
+        else:
+            rc, *_ = O.run(op, r_c, th, u, n, src_steps=ost, timing=True)
         if rc != 0:
             raise RuntimeError(f"oracle run failed ({rc})")
         return time.perf_counter() - t0
@@ -168,7 +178,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--timing-steps", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--integrator", choices=["rk4", "rk3"], default="rk4")
+    ap.add_argument("--integrator", choices=list(INTEGRATORS), default="rk4")
     ap.add_argument("--workload", choices=list(WORKLOADS), default="single_16384x64_plunge",
                     help="the single solve timed (the default is the headline configuration; the a=0.99 "
                          "ladder grids are larger single solves of the same path)")
@@ -197,7 +207,7 @@ def main():
         torch.cuda.synchronize()
         log("bench: copy calibration, 4 x (2 GiB read + 2 GiB written)")
         del x, y
-    integ = 1 if args.integrator == "rk4" else 0
+    integ = INTEGRATORS[args.integrator]
     maker, size = WORKLOADS[args.workload]
     p = getattr(workloads, maker)(*(size or ()), integrator=integ)
     need = args.warmup + args.steps + args.timing_steps
@@ -244,13 +254,13 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "N": p.N, "M": p.M, "rho": [p.xmin, p.xmax],
                    "a": p.a, "m": p.m, "dt_over_drho": 0.5,
-                   "integrator": {"rk4": "classic RK4", "rk3": "SSP-RK3"}[args.integrator],
+                   "integrator": INTEGRATOR_NAMES[args.integrator],
                    "t_final": p.T,
                    "source": "seeded plunge, mass ratio 1e-3, inclination 30 deg, from 1.02 r_isco",
                    "extraction_rho": p.xmin + p.j_extract * p.dx},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_stage (fused RK stage)", "bytes_per_point_update": b_alg,
+                     "kernel": "k_so2_half + k_so2_full (edges fused)" if args.integrator == "so2" else "k_stage (fused RK stage)", "bytes_per_point_update": b_alg,
                      "avg_launch_us": per_launch_s * 1e6},
     }
     if not args.no_batch:

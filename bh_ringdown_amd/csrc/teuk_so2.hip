@@ -4,16 +4,16 @@
 // The reference runs nine kernels and a host read-back per step
 // (Old/body.cu:121-243): average to the half points, source, rhs, half
 // update, theta edges; average back, source, rhs, full update, theta edges,
-// rho edges.  Here one step is four launches:
+// rho edges.  Here one step is two launches:
 //   k_so2_half     averages, right-hand side and half update at the half
 //                  points rho_{a-1/2}, straight from the grid-point fields
 //                  (Old/kern.cu:196-303);
-//   k_so2_half_bc  the half-point fields' theta edge rows (:305-340);
-//   k_so2_full     re-averages, right-hand side and full update at the grid
-//                  points, in place (:346-453);
-//   k_so2_bc       theta edge rows, then rho edge columns, then the time
-//                  integrals of Psi at the extraction column (:456-522 and
-//                  Old/body.cu:199-216).
+//   k_so2_full     re-averages (with the half-point theta edge rows of
+//                  :305-340 formed on the fly), right-hand side and full
+//                  update at the grid points in place (:346-453), then theta
+//                  edge rows, rho edge columns (:456-522) and the time
+//                  integrals of Psi at the extraction column
+//                  (Old/body.cu:199-216), the edges through LDS.
 // Each thread owns one point; a wavefront covers 64 consecutive rho columns
 // of one theta row, so every access is coalesced; the neighbour reads (one
 // column to the side, one row up and down) are cache hits.  The coefficient
@@ -142,127 +142,114 @@ __global__ __launch_bounds__(256) void k_so2_half(Grid g, DevState d, int has_sr
   for (int f = 0; f < 4; f++) Xh[f * NM + c] = teuk_pt::axpy(h, r[f], v0[f]);
 }
 
-// Theta edge rows of the half-point fields, every column (Old/kern.cu:305-340).
-// Column 0 of X_h is never written by the half step and stays zero.
-__global__ __launch_bounds__(256) void k_so2_half_bc(Grid g, DevState d) {
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  const int N = g.N, M = g.M;
-  if (a >= N) return;
-  const size_t NM = (size_t)N * M;
-  double *__restrict__ Xh = d.C;
-#pragma unroll
-  for (int f = 0; f < 4; f++) {
-    double *p = Xh + f * NM + a;
-    const double lo = g.even_m ? teuk_pt::so2_edge(p[(size_t)N], p[2 * (size_t)N]) : 0.0;
-    const double hi = g.even_m ? teuk_pt::so2_edge(p[(size_t)(M - 2) * N], p[(size_t)(M - 3) * N]) : 0.0;
-    p[0] = lo;
-    p[(size_t)(M - 1) * N] = hi;
-  }
-}
-
-// Full step: X += dt L_c with X_c = avg(X_h[a], X_h[a+1]), rows 1..M-2,
-// 1 <= a < N-1 (Old/kern.cu:346-453).  Reads X_h, updates X in place (each
-// thread reads only its own point of X).
+// Full step, edges and extraction in one pass (Old/kern.cu:305-522 and
+// Old/body.cu:199-216).  A block holds 64 columns x 4 rows.
+//  1. Interior points (rows 1..M-2, 1 <= a < N-1): X += dt L_c with
+//     X_c = avg(X_h[a], X_h[a+1]); the half-point fields' theta edge rows
+//     (kernel_boundary1, :305-340), needed only here, are formed on the fly
+//     from rows 1, 2 / M-2, M-3 -- the same operations in the same order.
+//     Each thread reads only its own point of X, so X is updated in place.
+//  2. Theta edge rows of X, 1 <= a <= N-2 (:456-497), from the block's new
+//     rows 1, 2 / M-2, M-3 through LDS (the first and last block rows hold
+//     them; column N-1 is the rho fill's).
+//  3. Rho edge columns of every row (:499-522), from the block's columns
+//     1, 2 / N-3, N-2 (the first and last block columns), after 2.
+//  4. The thread at the extraction column advances its row's time integrals
+//     from the final value and snapshots them every Fr steps.
 __global__ __launch_bounds__(256) void k_so2_full(Grid g, DevState d, int has_src, int k_rel) {
-  const int a = blockIdx.x * kWave + threadIdx.x;
-  const int b = blockIdx.y * kSo2Rows + threadIdx.y;
+  __shared__ double tile[4][kSo2Rows][kWave];
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int a = blockIdx.x * kWave + tx;
+  const int b = blockIdx.y * kSo2Rows + ty;
   const int N = g.N, M = g.M;
-  if (a < 1 || a >= N - 1 || b < 1 || b >= M - 1) return;
   const size_t NM = (size_t)N * M;
   const double *__restrict__ Xh = d.C;
   double *__restrict__ X = d.U;
   const size_t c = (size_t)b * N + a;
-  double qr[3], qi[3];
+  const bool inner = a >= 1 && a < N - 1 && b >= 1 && b < M - 1;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  if (inner) {
+    // X_h at row rr, column col, with the theta edge rows of kernel_boundary1
+    auto xh = [&](int f, int rr, int col) -> double {
+      const double *p = Xh + f * NM + col;
+      if (rr == 0) return g.even_m ? teuk_pt::so2_edge(p[(size_t)N], p[2 * (size_t)N]) : 0.0;
+      if (rr == M - 1) return g.even_m ? teuk_pt::so2_edge(p[(size_t)(M - 2) * N], p[(size_t)(M - 3) * N]) : 0.0;
+      return p[(size_t)rr * N];
+    };
+    double qr[3], qi[3];
 #pragma unroll
-  for (int i = 0; i < 3; i++) {
-    const size_t ci = c + (size_t)(i - 1) * N;
-    qr[i] = teuk_pt::so2_avg(Xh[ci], Xh[ci + 1]);
-    qi[i] = teuk_pt::so2_avg(Xh[NM + ci], Xh[NM + ci + 1]);
-  }
-  const double pr = teuk_pt::so2_avg(Xh[2 * NM + c], Xh[2 * NM + c + 1]);
-  const double pi = teuk_pt::so2_avg(Xh[3 * NM + c], Xh[3 * NM + c + 1]);
-  double dxv[4];
+    for (int i = 0; i < 3; i++) {
+      qr[i] = teuk_pt::so2_avg(xh(0, b + i - 1, a), xh(0, b + i - 1, a + 1));
+      qi[i] = teuk_pt::so2_avg(xh(1, b + i - 1, a), xh(1, b + i - 1, a + 1));
+    }
+    const double pr = teuk_pt::so2_avg(Xh[2 * NM + c], Xh[2 * NM + c + 1]);
+    const double pi = teuk_pt::so2_avg(Xh[3 * NM + c], Xh[3 * NM + c + 1]);
+    double dxv[4];
 #pragma unroll
-  for (int f = 0; f < 4; f++) dxv[f] = teuk_pt::so2_dx(Xh[f * NM + c + 1], Xh[f * NM + c], g.dx);
-  double sr = 0.0, si = 0.0;
-  if (has_src) so2_source(g, d, 2 * (d.ctl->k + k_rel) + 1, b, a, sr, si);
-  double r[4];
-  so2_rhs_point(g, d, d.ktab, d.dbdx, b, a, qr, qi, pr, pi, dxv, sr, si, r);
-#pragma unroll
-  for (int f = 0; f < 4; f++) X[f * NM + c] = teuk_pt::axpy(g.dt, r[f], X[f * NM + c]);
-}
-
-// Edges and extraction.  Blocks 0 .. nb-2: theta edge rows of columns
-// 1 <= a <= N-2 (Old/kern.cu:456-497; its column N-1 value is overwritten by
-// the rho fill).  Last block, one thread per theta row b: the rho edge
-// columns of row b (:499-522), taking row 0 / M-1 values at columns 1, 2,
-// N-3, N-2 from the theta fill it recomputes itself (so no thread waits for
-// another), then the time integrals of Psi at the extraction column
-// (Old/body.cu:199-216) and the Fr-step snapshot.
-__global__ __launch_bounds__(256) void k_so2_bc(Grid g, DevState d, int k_rel) {
-  const int N = g.N, M = g.M;
-  const size_t NM = (size_t)N * M;
-  double *__restrict__ X = d.U;
-  const int t = threadIdx.x;
-  if (blockIdx.x + 1 < gridDim.x) {
-    const int a = 1 + blockIdx.x * blockDim.x + t;
-    if (a > N - 2) return;
+    for (int f = 0; f < 4; f++) dxv[f] = teuk_pt::so2_dx(Xh[f * NM + c + 1], Xh[f * NM + c], g.dx);
+    double sr = 0.0, si = 0.0;
+    if (has_src) so2_source(g, d, 2 * (d.ctl->k + k_rel) + 1, b, a, sr, si);
+    double r[4];
+    so2_rhs_point(g, d, d.ktab, d.dbdx, b, a, qr, qi, pr, pi, dxv, sr, si, r);
 #pragma unroll
     for (int f = 0; f < 4; f++) {
-      double *p = X + f * NM + a;
-      const double lo = g.even_m ? teuk_pt::so2_edge(p[(size_t)N], p[2 * (size_t)N]) : 0.0;
-      const double hi = g.even_m ? teuk_pt::so2_edge(p[(size_t)(M - 2) * N], p[(size_t)(M - 3) * N]) : 0.0;
-      p[0] = lo;
-      p[(size_t)(M - 1) * N] = hi;
+      v[f] = teuk_pt::axpy(g.dt, r[f], X[f * NM + c]);
+      X[f * NM + c] = v[f];
     }
-    return;
-  }
-  const int b = t;
-  if (b >= M) return;
-  // Final value of (b, col) for 1 <= col <= N-2: the theta fill on the edge
-  // rows, the updated field elsewhere.
-  auto val = [&](int f, int col) -> double {
-    const double *p = X + f * NM + col;
-    if (b == 0) return g.even_m ? teuk_pt::so2_edge(p[(size_t)N], p[2 * (size_t)N]) : 0.0;
-    if (b == M - 1) return g.even_m ? teuk_pt::so2_edge(p[(size_t)(M - 2) * N], p[(size_t)(M - 3) * N]) : 0.0;
-    return p[(size_t)b * N];
-  };
-  double e0[4], e1[4], ej[4];
-  const int je = g.je;
-#pragma unroll
-  for (int f = 0; f < 4; f++) {
-    e0[f] = teuk_pt::so2_edge(val(f, 1), val(f, 2));
-    e1[f] = teuk_pt::so2_edge(val(f, N - 2), val(f, N - 3));
-    ej[f] = je == 0 ? e0[f] : (je == N - 1 ? e1[f] : val(f, je));
   }
 #pragma unroll
-  for (int f = 0; f < 4; f++) {
-    X[f * NM + (size_t)b * N] = e0[f];
-    X[f * NM + (size_t)b * N + N - 1] = e1[f];
+  for (int f = 0; f < 4; f++) tile[f][ty][tx] = v[f];
+  __syncthreads();
+  // 2. theta edge rows
+  const bool theta_edge = (b == 0 || b == M - 1) && a >= 1 && a <= N - 2;
+  if (theta_edge) {
+    const int t1 = b == 0 ? 1 : ty - 1, t2 = b == 0 ? 2 : ty - 2;
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+      v[f] = g.even_m ? teuk_pt::so2_edge(tile[f][t1][tx], tile[f][t2][tx]) : 0.0;
+      X[f * NM + c] = v[f];
+    }
   }
-  // Old/body.cu:209-216: rectangle-rule integrals, every step.
-  const Ext &ext = d.ext;
-  Ctl *ctl = d.ctl;
-  const int kstep = ctl->k + k_rel;
-  const double dt = g.dt;
-  const double q0 = ej[0], q1 = ej[1];
-  const double ir = ext.integ[b] + q0 * dt;
-  const double ii = ext.integ[M + b] + q1 * dt;
-  const double dr = ext.integ[2 * M + b] + ir * dt;
-  const double di = ext.integ[3 * M + b] + ii * dt;
-  ext.integ[b] = ir;
-  ext.integ[M + b] = ii;
-  ext.integ[2 * M + b] = dr;
-  ext.integ[3 * M + b] = di;
-  if (!isfinite(q0) || !isfinite(q1)) ctl->status = 1;
-  if ((kstep % g.Fr) == 0 && kstep / g.Fr < g.nrec_cap) {
-    double *sn = ext.snap + (size_t)(kstep / g.Fr) * 6 * M;
-    sn[b] = q0;
-    sn[M + b] = q1;
-    sn[2 * M + b] = ir;
-    sn[3 * M + b] = ii;
-    sn[4 * M + b] = dr;
-    sn[5 * M + b] = di;
+  __syncthreads();
+  if (theta_edge) {
+#pragma unroll
+    for (int f = 0; f < 4; f++) tile[f][ty][tx] = v[f];
+  }
+  __syncthreads();
+  // 3. rho edge columns
+  if (a == 0 || a == N - 1) {
+    const int t1 = a == 0 ? 1 : tx - 1, t2 = a == 0 ? 2 : tx - 2;
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+      v[f] = teuk_pt::so2_edge(tile[f][ty][t1], tile[f][ty][t2]);
+      X[f * NM + c] = v[f];
+    }
+  }
+  // 4. extraction (Old/body.cu:209-216), every step
+  if (a == g.je) {
+    const Ext &ext = d.ext;
+    Ctl *ctl = d.ctl;
+    const int kstep = ctl->k + k_rel;
+    const double dt = g.dt;
+    const double q0 = v[0], q1 = v[1];
+    const double ir = ext.integ[b] + q0 * dt;
+    const double ii = ext.integ[M + b] + q1 * dt;
+    const double dr = ext.integ[2 * M + b] + ir * dt;
+    const double di = ext.integ[3 * M + b] + ii * dt;
+    ext.integ[b] = ir;
+    ext.integ[M + b] = ii;
+    ext.integ[2 * M + b] = dr;
+    ext.integ[3 * M + b] = di;
+    if (!isfinite(q0) || !isfinite(q1)) ctl->status = 1;
+    if ((kstep % g.Fr) == 0 && kstep / g.Fr < g.nrec_cap) {
+      double *sn = ext.snap + (size_t)(kstep / g.Fr) * 6 * M;
+      sn[b] = q0;
+      sn[M + b] = q1;
+      sn[2 * M + b] = ir;
+      sn[3 * M + b] = ii;
+      sn[4 * M + b] = dr;
+      sn[5 * M + b] = di;
+    }
   }
 }
 
@@ -283,10 +270,7 @@ hipError_t launch_so2_step(const Grid &g, const DevState &d, bool has_src, int k
   const dim3 grid(g.N / kWave, g.M / kSo2Rows), block(kWave, kSo2Rows);
   const int hs = has_src ? 1 : 0;
   hipLaunchKernelGGL(k_so2_half, grid, block, 0, s, g, d, hs, k_rel);
-  hipLaunchKernelGGL(k_so2_half_bc, dim3((g.N + 255) / 256), dim3(256), 0, s, g, d);
   hipLaunchKernelGGL(k_so2_full, grid, block, 0, s, g, d, hs, k_rel);
-  const int nb_theta = (g.N - 2 + 255) / 256;
-  hipLaunchKernelGGL(k_so2_bc, dim3(nb_theta + 1), dim3(256), 0, s, g, d, k_rel);
   return hipGetLastError();
 }
 

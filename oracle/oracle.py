@@ -207,7 +207,7 @@ def step_so2(p, coef_h, coef_c, theta, u, src_h=None, src_c=None, work=None):
     return u
 
 
-def run_so2(p, r_h, r_c, theta, u, nsteps, src_steps=None, want_h=True):
+def run_so2(p, r_h, r_c, theta, u, nsteps, src_steps=None, want_h=True, timing=False):
     """Staggered-scheme run (two source entries per step); as run()."""
     M = p.M
     psi = np.zeros((nsteps, 2, M))
@@ -218,7 +218,7 @@ def run_so2(p, r_h, r_c, theta, u, nsteps, src_steps=None, want_h=True):
         assert len(src_steps) >= 2 * nsteps
         arr = (SrcStep * len(src_steps))(*src_steps)
         steps = ctypes.cast(arr, ctypes.POINTER(SrcStep))
-    rc = lib().orc_run_so2(ctypes.byref(p), _ptr(r_h), _ptr(r_c), _ptr(theta), _ptr(u), nsteps, steps,
+    rc = lib(timing).orc_run_so2(ctypes.byref(p), _ptr(r_h), _ptr(r_c), _ptr(theta), _ptr(u), nsteps, steps,
                            _ptr(psi), _ptr(h) if h is not None else None, _ptr(dedt))
     return rc, psi, h, dedt
 

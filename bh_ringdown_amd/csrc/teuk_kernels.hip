@@ -170,6 +170,8 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
   const double dt = g.dt;
   const double over_dx = 1.0 / g.dx, over_dth = 1.0 / g.dtheta;
   const double kof = teuk_pt::ko_factor(dt);
+  // RK4 weights, once per chunk (not per row inside the lane-divergent body)
+  const double h2 = 0.5 * dt, h6 = dt / 6.0, h3 = dt / 3.0;
   const double wx = kD1 * over_dx;
   const double wy = 0.05 * kThird * over_dth;
   const double wyy = 0.05 * kThird * kThird * over_dth * over_dth;
@@ -373,7 +375,6 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
           for (int f = 0; f < 4; f++) v[f] = teuk_pt::rk3_s3(late.base[f], u[f], dt, r[f], kThird);
         }
       } else {
-        const double h2 = 0.5 * dt, h6 = dt / 6.0, h3 = dt / 3.0;
         if (STAGE == 1) {
 #pragma unroll
           for (int f = 0; f < 4; f++) {

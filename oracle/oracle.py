@@ -128,7 +128,7 @@ def bc(p, u):
 
 def step(p, coef, theta, u, src=None, work=None):
     if work is None:
-        work = np.zeros((12, p.M, p.N))
+        work = np.zeros((16, p.M, p.N))
     lib().orc_step(ctypes.byref(p), _ptr(coef), _ptr(theta), _ptr(u),
                    _ptr(src) if src is not None else None, _ptr(work))
     return u

@@ -361,15 +361,17 @@ void orc_bc_rho(const orc_params *p, double *u) {
  * :602-620 (u3, in place).  Updates touch 3<=b<M-3, 3<=a<N-3 only; the
  * boundary fills then define every other point.
  * RK4 (classic, same RHS, source frozen over the step like the reference)
- * is the alternative integrator. */
+ * is the alternative integrator, written with its stage values (no
+ * accumulator).  work: 16 planes. */
 void orc_step(const orc_params *p, const double *coef, const double *theta,
               double *u, const double *src, double *work) {
   const int N = p->N, M = p->M;
   const size_t NM = plane(p);
   const double dt = p->dt;
-  double *uc = work;             /* 4 planes */
+  double *uc = work;             /* 4 planes (RK4: u1) */
   double *rhs = work + 4 * NM;   /* 4 planes */
-  double *acc = work + 8 * NM;   /* 4 planes (RK4) */
+  double *u2 = work + 8 * NM;    /* 4 planes (RK4) */
+  double *u3 = work + 12 * NM;   /* 4 planes (RK4) */
   const int nth = p->nthreads > 1 ? p->nthreads : 1;
   auto interior = [&](auto &&fn) {
 #pragma omp parallel for collapse(2) schedule(static) num_threads(nth) if (nth > 1)
@@ -388,18 +390,20 @@ void orc_step(const orc_params *p, const double *coef, const double *theta,
     interior([&](size_t c) { u[c] = teuk_pt::rk3_s3(u[c], uc[c], dt, rhs[c], kThird); });
     orc_bc_theta(p, u); orc_bc_rho(p, u);
   } else {
-    const double h2 = 0.5 * dt, h6 = dt / 6.0, h3 = dt / 3.0;
+    /* from the stage values: u1, u2, u3, then
+     * u + dt/6 (L(u) + 2 L(u1) + 2 L(u2) + L(u3)) = (u1 + 2 u2 + u3 - u)/3 + dt/6 L(u3) */
+    const double h2 = 0.5 * dt, h6 = dt / 6.0;
     orc_rhs(p, coef, theta, u, src, rhs);
-    interior([&](size_t c) { uc[c] = teuk_pt::axpy(h2, rhs[c], u[c]); acc[c] = teuk_pt::axpy(h6, rhs[c], u[c]); });
+    interior([&](size_t c) { uc[c] = teuk_pt::axpy(h2, rhs[c], u[c]); });
     orc_bc_theta(p, uc); orc_bc_rho(p, uc);
     orc_rhs(p, coef, theta, uc, src, rhs);
-    interior([&](size_t c) { uc[c] = teuk_pt::axpy(h2, rhs[c], u[c]); acc[c] = teuk_pt::axpy(h3, rhs[c], acc[c]); });
-    orc_bc_theta(p, uc); orc_bc_rho(p, uc);
-    orc_rhs(p, coef, theta, uc, src, rhs);
-    interior([&](size_t c) { uc[c] = teuk_pt::axpy(dt, rhs[c], u[c]); acc[c] = teuk_pt::axpy(h3, rhs[c], acc[c]); });
-    orc_bc_theta(p, uc); orc_bc_rho(p, uc);
-    orc_rhs(p, coef, theta, uc, src, rhs);
-    interior([&](size_t c) { u[c] = teuk_pt::axpy(h6, rhs[c], acc[c]); });
+    interior([&](size_t c) { u2[c] = teuk_pt::axpy(h2, rhs[c], u[c]); });
+    orc_bc_theta(p, u2); orc_bc_rho(p, u2);
+    orc_rhs(p, coef, theta, u2, src, rhs);
+    interior([&](size_t c) { u3[c] = teuk_pt::axpy(dt, rhs[c], u[c]); });
+    orc_bc_theta(p, u3); orc_bc_rho(p, u3);
+    orc_rhs(p, coef, theta, u3, src, rhs);
+    interior([&](size_t c) { u[c] = teuk_pt::rk4_last(u[c], uc[c], u2[c], u3[c], h6, rhs[c]); });
     orc_bc_theta(p, u); orc_bc_rho(p, u);
   }
 }
@@ -453,7 +457,7 @@ int orc_run(const orc_params *p, const double *r_c, const double *theta,
             double *psi_hist, double *h_hist, double *dedt_hist) {
   const int N = p->N, M = p->M;
   const size_t NM = plane(p);
-  std::vector<double> coef(9 * NM), work(12 * NM);
+  std::vector<double> coef(9 * NM), work(16 * NM);
   std::vector<double> src;
   if (src_steps) src.assign(2 * NM, 0.0);
   orc_coeffs(p, theta, r_c, coef.data());

@@ -57,7 +57,7 @@ void orc_bc_theta(const orc_params *p, double *u);
 void orc_bc_rho(const orc_params *p, double *u);
 
 /* One full time step (body.cu:131-165), in place on u.  work must hold
- * 12*N*M doubles.  src (2*N*M: re plane then im plane) may be NULL. */
+ * 16*N*M doubles.  src (2*N*M: re plane then im plane) may be NULL. */
 void orc_step(const orc_params *p, const double *coef, const double *theta,
               double *u, const double *src, double *work);
 

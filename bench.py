@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 # Algorithmic bytes per point-update (state traffic of one sweep per stage):
 # SSP-RK3 64 + 96 + 96; classic RK4 512 = the accumulator form's
 # 96 + 160 + 160 + 96 (the kernels use the stage-value form, which moves
-# 64 + 96 + 96 + 160 = 416; the roofline keeps the 512 figure, not
+# 64 + 128 + 96 + 96 = 384; the roofline keeps the 512 figure, not
 # renormalised to the cheaper form);
 # the staggered second-order scheme 64 (half step: fields in, half-point
 # fields out) + 96 (full step: half-point fields in, fields in and out).

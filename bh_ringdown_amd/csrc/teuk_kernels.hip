@@ -100,40 +100,38 @@ __global__ __launch_bounds__(256) void k_init_dbdx(Grid g, const double *__restr
 //     STAGE 2: out = 1/4 (3 base + in + dt L(in))
 //     STAGE 3: out = 1/3 (base + 2 in + 2 dt L(in))        (out may == base)
 //   INTEG 1 (classic RK4 from its stage values u1, u2, u3):
-//     STAGE 1: u1 = in + dt/2 L(in)                          (U -> C)
-//     STAGE 2: u2 = base + dt/2 L(in)                        (C -> D)
-//     STAGE 3: u3 = base + dt L(in)                          (D -> A)
-//     STAGE 4: out = (u1 + 2 u2 + u3 - base)/3 + dt/6 L(in)  (A -> U, in place)
-//   16 doubles of state traffic per point in stages 1-3 together with 20 in
-//   stage 4 (64 + 96 + 96 + 160 B), against 96 + 160 + 160 + 96 B with an
-//   accumulator array.
+//     STAGE 1: u1 = in + dt/2 L(in)                                (U -> C)
+//     STAGE 2: u2 = base + dt/2 L(in);  w = (in + 2 u2 - base)/3   (C -> D, A)
+//     STAGE 3: u3 = base + dt L(in)                                (D -> C)
+//     STAGE 4: out = w + in/3 + dt/6 L(in)                         (C, A -> U)
+//   State traffic 64 + 128 + 96 + 96 = 384 B per point, against
+//   96 + 160 + 160 + 96 B with an accumulator array.
 // The last stage also integrates Psi at the extraction column twice in time
 // (body.cu:177-187) and snapshots it every Fr steps.
 #ifndef TEUK_STAGE_WAVES_PER_EU
 #define TEUK_STAGE_WAVES_PER_EU 2
 #endif
 
-// Buffers of one stage (the ping-pong between the state planes); s1, s2 are
-// RK4's u1 and u2, read by its last stage.
+// Buffers of one stage (the ping-pong between the state planes); w is RK4's
+// partial combination, written by its second stage and read by its last.
 struct StageIO {
   const double *in;
   const double *base;
   double *out;
-  const double *s1;
-  const double *s2;
+  double *w;
 };
 
 template <int INTEG, int STAGE>
 __device__ __forceinline__ StageIO stage_io(const DevState &d) {
   if (INTEG == 0) {
-    if (STAGE == 1) return {d.U, d.U, d.C, nullptr, nullptr};
-    if (STAGE == 2) return {d.C, d.U, d.D, nullptr, nullptr};
-    return {d.D, d.U, d.U, nullptr, nullptr};
+    if (STAGE == 1) return {d.U, d.U, d.C, nullptr};
+    if (STAGE == 2) return {d.C, d.U, d.D, nullptr};
+    return {d.D, d.U, d.U, nullptr};
   }
-  if (STAGE == 1) return {d.U, d.U, d.C, nullptr, nullptr};
-  if (STAGE == 2) return {d.C, d.U, d.D, nullptr, nullptr};
-  if (STAGE == 3) return {d.D, d.U, d.A, nullptr, nullptr};
-  return {d.A, d.U, d.U, d.C, d.D};
+  if (STAGE == 1) return {d.U, d.U, d.C, nullptr};
+  if (STAGE == 2) return {d.C, d.U, d.D, d.A};
+  if (STAGE == 3) return {d.D, d.U, d.C, nullptr};
+  return {d.C, d.U, d.U, d.A};
 }
 
 // One tile of one stage: the wavefront threadIdx.y of block-row `grp` of
@@ -153,8 +151,7 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
   const double *__restrict__ in = io.in;
   const double *base = io.base;
   double *out = io.out;
-  const double *s1p = io.s1;
-  const double *s2p = io.s2;
+  double *wp = io.w;
   const double *__restrict__ ktab = d.ktab;
   const double *__restrict__ rtab = d.rtab;
   const double *__restrict__ dbdx = d.dbdx;
@@ -256,7 +253,7 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
     double pr, pi, halo[4];
   };
   struct RowLate {
-    double base[4], s1[4], s2[4], dbdx, sr, si, rt[kNR];  // rt: the row's sin^2, cos, Y, cot
+    double base[4], w[4], dbdx, sr, si, rt[kNR];  // rt: the row's sin^2, cos, Y, cot
   };
   const int slot = lane < kHalo ? lane : kWave + lane;  // halo slot of lanes 0..7
   const int hcol = a0 - kHalo + slot;
@@ -269,8 +266,8 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
   // ... and every lane writes its halo value to LDS: a lane without a halo
   // column writes its own column's value (the same bits) to its own slot.
   const int hslot = halo_lane ? slot : lane + kHalo;
-  constexpr bool kBase = STAGE >= 2;
-  constexpr bool kStageVals = INTEG == 1 && STAGE == 4;
+  constexpr bool kBase = INTEG == 0 ? STAGE >= 2 : (STAGE == 2 || STAGE == 3);
+  constexpr bool kWIn = INTEG == 1 && STAGE == 4;
   auto load_row = [&](int b) {
     RowIn x;
     const unsigned c0 = 8u * (unsigned)(b * N);
@@ -286,9 +283,7 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
 #pragma unroll
     for (int f = 0; f < 4; f++) x.base[f] = kBase ? ld(base, f * pb + c) : 0.0;
 #pragma unroll
-    for (int f = 0; f < 4; f++) x.s1[f] = kStageVals ? ld(s1p, f * pb + c) : 0.0;
-#pragma unroll
-    for (int f = 0; f < 4; f++) x.s2[f] = kStageVals ? ld(s2p, f * pb + c) : 0.0;
+    for (int f = 0; f < 4; f++) x.w[f] = kWIn ? ld(wp, f * pb + c) : 0.0;
     x.dbdx = ld(dbdx, c);
 #pragma unroll
     for (int j = 0; j < kNR; j++) x.rt[j] = rtab[j * M + b];
@@ -385,13 +380,19 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
         if (STAGE == 1) {
 #pragma unroll
           for (int f = 0; f < 4; f++) v[f] = teuk_pt::axpy(h2, r[f], u[f]);
-        } else if (STAGE == 2 || STAGE == 3) {
-          const double h = (STAGE == 2) ? h2 : dt;
+        } else if (STAGE == 2) {
+          const unsigned c = 8u * (unsigned)(b * N + a);
 #pragma unroll
-          for (int f = 0; f < 4; f++) v[f] = teuk_pt::axpy(h, r[f], late.base[f]);
+          for (int f = 0; f < 4; f++) {
+            v[f] = teuk_pt::axpy(h2, r[f], late.base[f]);
+            st(wp, f * pb + c, teuk_pt::rk4_w(late.base[f], u[f], v[f]));
+          }
+        } else if (STAGE == 3) {
+#pragma unroll
+          for (int f = 0; f < 4; f++) v[f] = teuk_pt::axpy(dt, r[f], late.base[f]);
         } else {
 #pragma unroll
-          for (int f = 0; f < 4; f++) v[f] = teuk_pt::rk4_last(late.base[f], late.s1[f], late.s2[f], u[f], h6, r[f]);
+          for (int f = 0; f < 4; f++) v[f] = teuk_pt::rk4_last(late.w[f], u[f], h6, r[f]);
         }
       }
     }

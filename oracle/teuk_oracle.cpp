@@ -368,10 +368,10 @@ void orc_step(const orc_params *p, const double *coef, const double *theta,
   const int N = p->N, M = p->M;
   const size_t NM = plane(p);
   const double dt = p->dt;
-  double *uc = work;             /* 4 planes (RK4: u1) */
+  double *uc = work;             /* 4 planes (RK4: u1, then u3) */
   double *rhs = work + 4 * NM;   /* 4 planes */
   double *u2 = work + 8 * NM;    /* 4 planes (RK4) */
-  double *u3 = work + 12 * NM;   /* 4 planes (RK4) */
+  double *w = work + 12 * NM;    /* 4 planes (RK4) */
   const int nth = p->nthreads > 1 ? p->nthreads : 1;
   auto interior = [&](auto &&fn) {
 #pragma omp parallel for collapse(2) schedule(static) num_threads(nth) if (nth > 1)
@@ -390,20 +390,24 @@ void orc_step(const orc_params *p, const double *coef, const double *theta,
     interior([&](size_t c) { u[c] = teuk_pt::rk3_s3(u[c], uc[c], dt, rhs[c], kThird); });
     orc_bc_theta(p, u); orc_bc_rho(p, u);
   } else {
-    /* from the stage values: u1, u2, u3, then
-     * u + dt/6 (L(u) + 2 L(u1) + 2 L(u2) + L(u3)) = (u1 + 2 u2 + u3 - u)/3 + dt/6 L(u3) */
+    /* from the stage values: u1, u2 with w = (u1 + 2 u2 - u)/3, u3, then
+     * u + dt/6 (L(u) + 2 L(u1) + 2 L(u2) + L(u3)) = w + u3/3 + dt/6 L(u3) */
+    double *u3 = uc;               /* u1 is dead once w holds it */
     const double h2 = 0.5 * dt, h6 = dt / 6.0;
     orc_rhs(p, coef, theta, u, src, rhs);
     interior([&](size_t c) { uc[c] = teuk_pt::axpy(h2, rhs[c], u[c]); });
     orc_bc_theta(p, uc); orc_bc_rho(p, uc);
     orc_rhs(p, coef, theta, uc, src, rhs);
-    interior([&](size_t c) { u2[c] = teuk_pt::axpy(h2, rhs[c], u[c]); });
+    interior([&](size_t c) {
+      u2[c] = teuk_pt::axpy(h2, rhs[c], u[c]);
+      w[c] = teuk_pt::rk4_w(u[c], uc[c], u2[c]);
+    });
     orc_bc_theta(p, u2); orc_bc_rho(p, u2);
     orc_rhs(p, coef, theta, u2, src, rhs);
     interior([&](size_t c) { u3[c] = teuk_pt::axpy(dt, rhs[c], u[c]); });
     orc_bc_theta(p, u3); orc_bc_rho(p, u3);
     orc_rhs(p, coef, theta, u3, src, rhs);
-    interior([&](size_t c) { u[c] = teuk_pt::rk4_last(u[c], uc[c], u2[c], u3[c], h6, rhs[c]); });
+    interior([&](size_t c) { u[c] = teuk_pt::rk4_last(w[c], u3[c], h6, rhs[c]); });
     orc_bc_theta(p, u); orc_bc_rho(p, u);
   }
 }

@@ -120,13 +120,14 @@ TEUK_HD double rk3_s3(double base, double u, double dt, double r, double third) 
   return third * fma_(2.0 * dt, r, fma_(2.0, u, base));
 }
 // Classic RK4 from its stage values: u1 = u + dt/2 L(u), u2 = u + dt/2 L(u1),
-// u3 = u + dt L(u2) (each an axpy), and
-//   u(t + dt) = (u1 + 2 u2 + u3 - u) / 3 + dt/6 L(u3),
+// u3 = u + dt L(u2) (each an axpy), w = (u1 + 2 u2 - u)/3 formed with u2, and
+//   u(t + dt) = w + u3/3 + dt/6 L(u3),
 // which is u + dt/6 L(u) + dt/3 L(u1) + dt/3 L(u2) + dt/6 L(u3) without an
-// accumulator array.
+// accumulator array (and without u in the last stage).
 TEUK_HD double axpy(double h, double r, double y) { return fma_(h, r, y); }
-TEUK_HD double rk4_last(double u, double u1, double u2, double u3, double h6, double r) {
-  return fma_(h6, r, (fma_(2.0, u2, u1 + u3) - u) * (1.0 / 3.0));
+TEUK_HD double rk4_w(double u, double u1, double u2) { return (fma_(2.0, u2, u1) - u) * (1.0 / 3.0); }
+TEUK_HD double rk4_last(double w, double u3, double h6, double r) {
+  return fma_(h6, r, fma_(u3, 1.0 / 3.0, w));
 }
 
 // Staggered second-order scheme of TD_GPU_code/Old/kern.cu: fields averaged

@@ -246,7 +246,7 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
 
   // Global inputs of a row.  The stencil inputs (p and the halo columns) are
   // fetched one row ahead; the inputs used only after the LDS stencils
-  // (stage base, RK4 accumulator, dbdx, source patch, row values) are issued at the top
+  // (stage base, RK4 partial combination, dbdx, source patch, row values) are issued at the top
   // of their own row, before the next row's prefetch, so that waiting for
   // them never waits for the prefetch (loads retire in issue order).
   struct RowIn {

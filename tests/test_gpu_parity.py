@@ -130,7 +130,7 @@ def test_modes_even_and_odd(m):
     check(make(256, 32, -50.0, 50.0, 0.7, m, 60), 60)
 
 
-@pytest.mark.parametrize("a", [0.0, 0.3, 0.9, 0.99])
+@pytest.mark.parametrize("a", [0.0, 0.3, 0.9, 0.99, -0.5])
 def test_spins(a):
     check(make(512, 32, -50.0, 50.0, a, 2, 60), 60)
 
@@ -194,3 +194,20 @@ def test_rejects_bad_sizes():
     with teuk.Solver(p) as s:
         with pytest.raises(teuk.TeukError):
             s.step(6)  # beyond the scheduled T
+
+
+@pytest.mark.parametrize("integrator", [1, 2])
+def test_non_finite_field_is_reported(integrator):
+    """A NaN planted in the initial data next to the extraction column reaches
+    it within a step; teuk_sync then reports TEUK_EDIVERGED (the reference
+    has no such check).  The kernels run to completion on the NaN data."""
+    p = make(256, 16, -50.0, 50.0, 0.5, 2, 8, integrator=integrator)
+    u0 = workloads.gaussian_pulse(p, 5, center=10.0)
+    u0[0, p.M // 2, p.N - 6] = np.nan
+    _, r_c, th = teuk.grid(p)
+    with teuk.Solver(p, r_c, th) as s:
+        s.set_fields(u0)
+        s.step(8)
+        with pytest.raises(teuk.TeukError) as e:
+            s.sync()
+        assert e.value.code == teuk.TEUK_EDIVERGED

@@ -250,22 +250,24 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
   // of their own row, before the next row's prefetch, so that waiting for
   // them never waits for the prefetch (loads retire in issue order).
   struct RowIn {
-    double pr, pi, halo[4];
+    double pr, pi, halo;   // halo: this lane's entry of the row's 4 x 8 halo values
   };
   struct RowLate {
     double base[4], w[4], dbdx, sr, si, rt[kNR];  // rt: the row's sin^2, cos, Y, cot
   };
-  const int slot = lane < kHalo ? lane : kWave + lane;  // halo slot of lanes 0..7
-  const int hcol = a0 - kHalo + slot;
-  const bool halo_lane = lane < 2 * kHalo && hcol >= 0 && hcol < N;
-  // Every load below is issued by all lanes from an in-bounds address (lanes
-  // without a halo column or outside the patch re-read their own column, a
-  // cache hit, and discard the value): no branches around loads, so the
-  // compiler's wait counts stay exact and never cover the prefetch.
-  const int hsafe = halo_lane ? hcol : a;
-  // ... and every lane writes its halo value to LDS: a lane without a halo
-  // column writes its own column's value (the same bits) to its own slot.
-  const int hslot = halo_lane ? slot : lane + kHalo;
+  // The 4 fields' halo columns (4 left, 4 right of the strip) of a row come
+  // from one load: lane l serves field (l / 8) % 4, halo slot l % 8 (lanes
+  // 32..63 repeat lanes 0..31).  A halo column outside the grid (first and
+  // last strip) reads the strip's own edge column instead; no update reads
+  // those slots.  Every load below is issued by all lanes from an in-bounds
+  // address (lanes outside the patch re-read their own column): no branches
+  // around loads, so the compiler's wait counts stay exact and never cover
+  // the prefetch.
+  const int hfield = (lane >> 3) & 3, hs = lane & 7;
+  const int hslot = hs < kHalo ? hs : kWave + hs;          // LDS slot of the halo value
+  const int hcol = a0 - kHalo + hslot;
+  const int hsafe = (hcol >= 0 && hcol < N) ? hcol : (hs < kHalo ? a0 : a0 + kWave - 1);
+  const unsigned hoff = (unsigned)hfield * pb + 8u * (unsigned)hsafe;
   constexpr bool kBase = INTEG == 0 ? STAGE >= 2 : (STAGE == 2 || STAGE == 3);
   constexpr bool kWIn = INTEG == 1 && STAGE == 4;
   auto load_row = [&](int b) {
@@ -273,8 +275,7 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
     const unsigned c0 = 8u * (unsigned)(b * N);
     x.pr = ld(in, 2 * pb + c0 + 8u * a);
     x.pi = ld(in, 3 * pb + c0 + 8u * a);
-#pragma unroll
-    for (int f = 0; f < 4; f++) x.halo[f] = ld(in, f * pb + c0 + 8u * hsafe);
+    x.halo = ld(in, hoff + c0);
     return x;
   };
   auto load_late = [&](int b) {
@@ -316,8 +317,7 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
     L[1 * kLdsCols + lane + kHalo] = qi0;
     L[2 * kLdsCols + lane + kHalo] = pr0;
     L[3 * kLdsCols + lane + kHalo] = pi0;
-#pragma unroll
-    for (int f = 0; f < 4; f++) L[f * kLdsCols + hslot] = cur.halo[f];
+    L[hfield * kLdsCols + hslot] = cur.halo;
     wave_sync();
 
     double v[4] = {0, 0, 0, 0};

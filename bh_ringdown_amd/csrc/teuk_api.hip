@@ -550,19 +550,34 @@ void teuk_destroy(teuk_solver *s) {
   delete s;
 }
 
+// On the device a state is two complex planes, Psi and Pi, each [M][N] of
+// (re, im) pairs (one 16-byte access per complex value); the ABI's layout is
+// four real planes q_re, q_im, p_re, p_im (kern.cu:10-13).
 int teuk_set_fields(teuk_solver *s, const double *u) {
   if (!s || !u) return TEUK_EINVAL;
-  const size_t bytes = 4 * (size_t)s->P.N * s->P.M * sizeof(double);
-  HIPCHK(hipMemcpyAsync(s->d.U, u, bytes, hipMemcpyHostToDevice, run_stream(s)));
+  const size_t NM = (size_t)s->P.N * s->P.M;
+  std::vector<double> dev(4 * NM);
+  for (size_t h = 0; h < 2; h++)
+    for (size_t c = 0; c < NM; c++) {
+      dev[h * 2 * NM + 2 * c] = u[(2 * h) * NM + c];
+      dev[h * 2 * NM + 2 * c + 1] = u[(2 * h + 1) * NM + c];
+    }
+  HIPCHK(hipMemcpyAsync(s->d.U, dev.data(), dev.size() * sizeof(double), hipMemcpyHostToDevice, run_stream(s)));
   HIPCHK(hipStreamSynchronize(run_stream(s)));
   return TEUK_OK;
 }
 
 int teuk_get_fields(teuk_solver *s, double *u) {
   if (!s || !u) return TEUK_EINVAL;
-  const size_t bytes = 4 * (size_t)s->P.N * s->P.M * sizeof(double);
-  HIPCHK(hipMemcpyAsync(u, s->d.U, bytes, hipMemcpyDeviceToHost, run_stream(s)));
+  const size_t NM = (size_t)s->P.N * s->P.M;
+  std::vector<double> dev(4 * NM);
+  HIPCHK(hipMemcpyAsync(dev.data(), s->d.U, dev.size() * sizeof(double), hipMemcpyDeviceToHost, run_stream(s)));
   HIPCHK(hipStreamSynchronize(run_stream(s)));
+  for (size_t h = 0; h < 2; h++)
+    for (size_t c = 0; c < NM; c++) {
+      u[(2 * h) * NM + c] = dev[h * 2 * NM + 2 * c];
+      u[(2 * h + 1) * NM + c] = dev[h * 2 * NM + 2 * c + 1];
+    }
   return TEUK_OK;
 }
 

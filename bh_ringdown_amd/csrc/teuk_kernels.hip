@@ -46,6 +46,13 @@ __device__ __forceinline__ double ld(const double *base, unsigned off) {
 __device__ __forceinline__ void st(double *base, unsigned off, double v) {
   *reinterpret_cast<double *>(reinterpret_cast<char *>(base) + off) = v;
 }
+// A complex value (re, im) of a state: 16 bytes, one load or store.
+__device__ __forceinline__ double2 ld2(const double *base, unsigned off) {
+  return *reinterpret_cast<const double2 *>(reinterpret_cast<const char *>(base) + off);
+}
+__device__ __forceinline__ void st2(double *base, unsigned off, double re, double im) {
+  *reinterpret_cast<double2 *>(reinterpret_cast<char *>(base) + off) = make_double2(re, im);
+}
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -143,7 +150,9 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
   __shared__ double lds[4][4][kLdsCols];
   const int N = g.N, M = g.M;
   const size_t NM = (size_t)N * M;
-  const unsigned pb = 8u * (unsigned)NM;   // bytes of one plane
+  // A state is two complex planes, Psi then Pi, each [M][N] of (re, im):
+  // field f of point c sits at byte (f / 2) cb + 16 c + 8 (f % 2).
+  const unsigned cb = 16u * (unsigned)NM;
   // The wavefront index is uniform: make that visible to the compiler so the
   // row index b is a scalar and the per-row tables are scalar loads.
   const int lane = threadIdx.x, w = __builtin_amdgcn_readfirstlane(threadIdx.y);
@@ -190,8 +199,9 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
   for (int i = 0; i < kRows + 6; i++) {
     const int r = b0 - 3 + i;
     if (r >= 0 && r < M) {
-      wqr[i] = in[(size_t)r * N + a];
-      wqi[i] = in[NM + (size_t)r * N + a];
+      const double2 q = ld2(in, 16u * (unsigned)(r * N + a));
+      wqr[i] = q.x;
+      wqi[i] = q.y;
     } else {
       wqr[i] = 0.0;
       wqi[i] = 0.0;
@@ -239,9 +249,9 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
   // Store one finished row (no conditions: the compiler's memory wait counts
   // stay exact across rows).
   auto store_row = [&](int b, const double (&v)[4]) {
-    const unsigned c = 8u * (unsigned)(b * N + a);
-#pragma unroll
-    for (int f = 0; f < 4; f++) st(out, f * pb + c, v[f]);
+    const unsigned c = 16u * (unsigned)(b * N + a);
+    st2(out, c, v[0], v[1]);
+    st2(out, cb + c, v[2], v[3]);
   };
 
   // Global inputs of a row.  The stencil inputs (p and the halo columns) are
@@ -250,41 +260,49 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
   // of their own row, before the next row's prefetch, so that waiting for
   // them never waits for the prefetch (loads retire in issue order).
   struct RowIn {
-    double pr, pi, halo;   // halo: this lane's entry of the row's 4 x 8 halo values
+    double pr, pi;
+    double2 halo;   // this lane's entry of the row's 2 x 8 complex halo values
   };
   struct RowLate {
     double base[4], w[4], dbdx, sr, si, rt[kNR];  // rt: the row's sin^2, cos, Y, cot
   };
-  // The 4 fields' halo columns (4 left, 4 right of the strip) of a row come
-  // from one load: lane l serves field (l / 8) % 4, halo slot l % 8 (lanes
-  // 32..63 repeat lanes 0..31).  A halo column outside the grid (first and
-  // last strip) reads the strip's own edge column instead; no update reads
-  // those slots.  Every load below is issued by all lanes from an in-bounds
-  // address (lanes outside the patch re-read their own column): no branches
-  // around loads, so the compiler's wait counts stay exact and never cover
-  // the prefetch.
-  const int hfield = (lane >> 3) & 3, hs = lane & 7;
+  // The halo columns (4 left, 4 right of the strip) of a row, both complex
+  // fields, come from one load: lane l serves plane (l / 8) % 2 (Psi, Pi),
+  // halo slot l % 8 (lanes 16..63 repeat lanes 0..15).  A halo column
+  // outside the grid (first and last strip) reads the strip's own edge column
+  // instead; no update reads those slots.  Every load below is issued by all
+  // lanes from an in-bounds address (lanes outside the patch re-read their
+  // own column): no branches around loads, so the compiler's wait counts
+  // stay exact and never cover the prefetch.
+  const int hplane = (lane >> 3) & 1, hs = lane & 7;
   const int hslot = hs < kHalo ? hs : kWave + hs;          // LDS slot of the halo value
   const int hcol = a0 - kHalo + hslot;
   const int hsafe = (hcol >= 0 && hcol < N) ? hcol : (hs < kHalo ? a0 : a0 + kWave - 1);
-  const unsigned hoff = (unsigned)hfield * pb + 8u * (unsigned)hsafe;
+  const unsigned hoff = (unsigned)hplane * cb + 16u * (unsigned)hsafe;
   constexpr bool kBase = INTEG == 0 ? STAGE >= 2 : (STAGE == 2 || STAGE == 3);
   constexpr bool kWIn = INTEG == 1 && STAGE == 4;
   auto load_row = [&](int b) {
     RowIn x;
-    const unsigned c0 = 8u * (unsigned)(b * N);
-    x.pr = ld(in, 2 * pb + c0 + 8u * a);
-    x.pi = ld(in, 3 * pb + c0 + 8u * a);
-    x.halo = ld(in, hoff + c0);
+    const unsigned c0 = 16u * (unsigned)(b * N);
+    const double2 p = ld2(in, cb + c0 + 16u * a);
+    x.pr = p.x;
+    x.pi = p.y;
+    x.halo = ld2(in, hoff + c0);
     return x;
   };
   auto load_late = [&](int b) {
     RowLate x;
     const unsigned c = 8u * (unsigned)(b * N + a);
+    const unsigned c2 = 16u * (unsigned)(b * N + a);
 #pragma unroll
-    for (int f = 0; f < 4; f++) x.base[f] = kBase ? ld(base, f * pb + c) : 0.0;
-#pragma unroll
-    for (int f = 0; f < 4; f++) x.w[f] = kWIn ? ld(wp, f * pb + c) : 0.0;
+    for (int h = 0; h < 2; h++) {
+      const double2 bv = kBase ? ld2(base, h * cb + c2) : make_double2(0.0, 0.0);
+      x.base[2 * h] = bv.x;
+      x.base[2 * h + 1] = bv.y;
+      const double2 wv = kWIn ? ld2(wp, h * cb + c2) : make_double2(0.0, 0.0);
+      x.w[2 * h] = wv.x;
+      x.w[2 * h + 1] = wv.y;
+    }
     x.dbdx = ld(dbdx, c);
 #pragma unroll
     for (int j = 0; j < kNR; j++) x.rt[j] = rtab[j * M + b];
@@ -317,7 +335,8 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
     L[1 * kLdsCols + lane + kHalo] = qi0;
     L[2 * kLdsCols + lane + kHalo] = pr0;
     L[3 * kLdsCols + lane + kHalo] = pi0;
-    L[hfield * kLdsCols + hslot] = cur.halo;
+    L[(2 * hplane) * kLdsCols + hslot] = cur.halo.x;
+    L[(2 * hplane + 1) * kLdsCols + hslot] = cur.halo.y;
     wave_sync();
 
     double v[4] = {0, 0, 0, 0};
@@ -381,12 +400,15 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
 #pragma unroll
           for (int f = 0; f < 4; f++) v[f] = teuk_pt::axpy(h2, r[f], u[f]);
         } else if (STAGE == 2) {
-          const unsigned c = 8u * (unsigned)(b * N + a);
+          const unsigned c = 16u * (unsigned)(b * N + a);
+          double wv[4];
 #pragma unroll
           for (int f = 0; f < 4; f++) {
             v[f] = teuk_pt::axpy(h2, r[f], late.base[f]);
-            st(wp, f * pb + c, teuk_pt::rk4_w(late.base[f], u[f], v[f]));
+            wv[f] = teuk_pt::rk4_w(late.base[f], u[f], v[f]);
           }
+          st2(wp, c, wv[0], wv[1]);
+          st2(wp, cb + c, wv[2], wv[3]);
         } else if (STAGE == 3) {
 #pragma unroll
           for (int f = 0; f < 4; f++) v[f] = teuk_pt::axpy(dt, r[f], late.base[f]);
@@ -462,8 +484,8 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
   // snapshots them every Fr steps.
   if (kLast && g.je >= a0 && g.je < a0 + kWave && a == g.je) {
     for (int b = b0; b < b0 + kRows; b++) {
-      const size_t c = (size_t)b * N + a;
-      const double q0 = out[c], q1 = out[NM + c];
+      const double2 q = ld2(out, 16u * (unsigned)(b * N + a));
+      const double q0 = q.x, q1 = q.y;
       const double ir = ext.integ[b] + q0 * dt;
       const double ii = ext.integ[M + b] + q1 * dt;
       const double dr = ext.integ[2 * M + b] + ir * dt;

@@ -31,6 +31,12 @@ using teuk_pt::Coef;
 
 constexpr int kSo2Rows = 4;   // theta rows per block (64 x 4 threads)
 
+// Field f (q_re, q_im, p_re, p_im) of point c: a state is two complex planes
+// (Psi, Pi), each [M][N] of (re, im).
+__device__ __forceinline__ size_t fidx(int f, size_t c, size_t NM) {
+  return (size_t)(f >> 1) * 2 * NM + 2 * c + (size_t)(f & 1);
+}
+
 __device__ __forceinline__ Coef so2_coef(const double *__restrict__ ktab, int N, int a, double sn2, double cs,
                                          double Y) {
   double K[kNK];
@@ -124,14 +130,14 @@ __global__ __launch_bounds__(256) void k_so2_half(Grid g, DevState d, int has_sr
 #pragma unroll
   for (int i = 0; i < 3; i++) {
     const size_t ci = c + (size_t)(i - 1) * N;
-    qr[i] = teuk_pt::so2_avg(X[ci], X[ci - 1]);
-    qi[i] = teuk_pt::so2_avg(X[NM + ci], X[NM + ci - 1]);
+    qr[i] = teuk_pt::so2_avg(X[fidx(0, ci, NM)], X[fidx(0, ci - 1, NM)]);
+    qi[i] = teuk_pt::so2_avg(X[fidx(1, ci, NM)], X[fidx(1, ci - 1, NM)]);
   }
-  const double pr = teuk_pt::so2_avg(X[2 * NM + c], X[2 * NM + c - 1]);
-  const double pi = teuk_pt::so2_avg(X[3 * NM + c], X[3 * NM + c - 1]);
+  const double pr = teuk_pt::so2_avg(X[fidx(2, c, NM)], X[fidx(2, c - 1, NM)]);
+  const double pi = teuk_pt::so2_avg(X[fidx(3, c, NM)], X[fidx(3, c - 1, NM)]);
   double dxv[4];
 #pragma unroll
-  for (int f = 0; f < 4; f++) dxv[f] = teuk_pt::so2_dx(X[f * NM + c], X[f * NM + c - 1], g.dx);
+  for (int f = 0; f < 4; f++) dxv[f] = teuk_pt::so2_dx(X[fidx(f, c, NM)], X[fidx(f, c - 1, NM)], g.dx);
   double sr = 0.0, si = 0.0;
   if (has_src) so2_source(g, d, 2 * (d.ctl->k + k_rel), b, a, sr, si);
   double r[4];
@@ -139,7 +145,7 @@ __global__ __launch_bounds__(256) void k_so2_half(Grid g, DevState d, int has_sr
   const double h = 0.5 * g.dt;
   const double v0[4] = {qr[1], qi[1], pr, pi};
 #pragma unroll
-  for (int f = 0; f < 4; f++) Xh[f * NM + c] = teuk_pt::axpy(h, r[f], v0[f]);
+  for (int f = 0; f < 4; f++) Xh[fidx(f, c, NM)] = teuk_pt::axpy(h, r[f], v0[f]);
 }
 
 // Full step, edges and extraction in one pass (Old/kern.cu:305-522 and
@@ -171,10 +177,10 @@ __global__ __launch_bounds__(256) void k_so2_full(Grid g, DevState d, int has_sr
   if (inner) {
     // X_h at row rr, column col, with the theta edge rows of kernel_boundary1
     auto xh = [&](int f, int rr, int col) -> double {
-      const double *p = Xh + f * NM + col;
-      if (rr == 0) return g.even_m ? teuk_pt::so2_edge(p[(size_t)N], p[2 * (size_t)N]) : 0.0;
-      if (rr == M - 1) return g.even_m ? teuk_pt::so2_edge(p[(size_t)(M - 2) * N], p[(size_t)(M - 3) * N]) : 0.0;
-      return p[(size_t)rr * N];
+      auto at = [&](int row) { return Xh[fidx(f, (size_t)row * N + col, NM)]; };
+      if (rr == 0) return g.even_m ? teuk_pt::so2_edge(at(1), at(2)) : 0.0;
+      if (rr == M - 1) return g.even_m ? teuk_pt::so2_edge(at(M - 2), at(M - 3)) : 0.0;
+      return at(rr);
     };
     double qr[3], qi[3];
 #pragma unroll
@@ -182,19 +188,19 @@ __global__ __launch_bounds__(256) void k_so2_full(Grid g, DevState d, int has_sr
       qr[i] = teuk_pt::so2_avg(xh(0, b + i - 1, a), xh(0, b + i - 1, a + 1));
       qi[i] = teuk_pt::so2_avg(xh(1, b + i - 1, a), xh(1, b + i - 1, a + 1));
     }
-    const double pr = teuk_pt::so2_avg(Xh[2 * NM + c], Xh[2 * NM + c + 1]);
-    const double pi = teuk_pt::so2_avg(Xh[3 * NM + c], Xh[3 * NM + c + 1]);
+    const double pr = teuk_pt::so2_avg(Xh[fidx(2, c, NM)], Xh[fidx(2, c + 1, NM)]);
+    const double pi = teuk_pt::so2_avg(Xh[fidx(3, c, NM)], Xh[fidx(3, c + 1, NM)]);
     double dxv[4];
 #pragma unroll
-    for (int f = 0; f < 4; f++) dxv[f] = teuk_pt::so2_dx(Xh[f * NM + c + 1], Xh[f * NM + c], g.dx);
+    for (int f = 0; f < 4; f++) dxv[f] = teuk_pt::so2_dx(Xh[fidx(f, c + 1, NM)], Xh[fidx(f, c, NM)], g.dx);
     double sr = 0.0, si = 0.0;
     if (has_src) so2_source(g, d, 2 * (d.ctl->k + k_rel) + 1, b, a, sr, si);
     double r[4];
     so2_rhs_point(g, d, d.ktab, d.dbdx, b, a, qr, qi, pr, pi, dxv, sr, si, r);
 #pragma unroll
     for (int f = 0; f < 4; f++) {
-      v[f] = teuk_pt::axpy(g.dt, r[f], X[f * NM + c]);
-      X[f * NM + c] = v[f];
+      v[f] = teuk_pt::axpy(g.dt, r[f], X[fidx(f, c, NM)]);
+      X[fidx(f, c, NM)] = v[f];
     }
   }
 #pragma unroll
@@ -207,7 +213,7 @@ __global__ __launch_bounds__(256) void k_so2_full(Grid g, DevState d, int has_sr
 #pragma unroll
     for (int f = 0; f < 4; f++) {
       v[f] = g.even_m ? teuk_pt::so2_edge(tile[f][t1][tx], tile[f][t2][tx]) : 0.0;
-      X[f * NM + c] = v[f];
+      X[fidx(f, c, NM)] = v[f];
     }
   }
   __syncthreads();
@@ -222,7 +228,7 @@ __global__ __launch_bounds__(256) void k_so2_full(Grid g, DevState d, int has_sr
 #pragma unroll
     for (int f = 0; f < 4; f++) {
       v[f] = teuk_pt::so2_edge(tile[f][ty][t1], tile[f][ty][t2]);
-      X[f * NM + c] = v[f];
+      X[fidx(f, c, NM)] = v[f];
     }
   }
   // 4. extraction (Old/body.cu:209-216), every step

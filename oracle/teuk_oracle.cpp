@@ -19,8 +19,12 @@
  *   orc_fact/orc_swsh/orc_project
  *                 bitwise against the reference proj.c compiled into
  *                 oracle/_ref, plus closed forms of -2Y_lm and orthonormality.
- *   orc_coeffs    parity unpinned: a restatement of kern.cu:132-158 in
- *                 factored form; no reference output exists to compare with.
+ *   orc_coeffs    a restatement of kern.cu:132-158 in factored form, pinned
+ *                 against Teukolsky's equation: the second-order operator the
+ *                 nine fields define equals the one derived symbolically from
+ *                 the s = -2 master equation in the compactified coordinates
+ *                 (Kerr azimuth, r^3 rescaling, hyperboloidal layer), to
+ *                 1e-11 inside and outside the layer; bb is the negative root.
  *   orc_rhs       stencil exactness on polynomials, KO annihilation of degree
  *                 <= 7, assembly against a direct evaluation on smooth fields.
  *   orc_bc_*      exact extrapolation formulas and ordering invariants.

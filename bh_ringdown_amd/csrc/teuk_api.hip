@@ -854,7 +854,6 @@ int teuk_set_tiling(teuk_solver *s, int waves_per_block, int waves_per_eu) {
   if (!s || s->batch || s->g.stag) return TEUK_EINVAL;
   if (!(waves_per_block == 1 || waves_per_block == 2 || waves_per_block == 4)) return TEUK_EINVAL;
   if (!(waves_per_eu == 2 || waves_per_eu == 4)) return TEUK_EINVAL;
-  if (s->P.M % (kRows * waves_per_block) != 0) return TEUK_EINVAL;
   s->tiling.wpb = waves_per_block;
   s->tiling.wpe = waves_per_eu;
   if (s->gexec) {
@@ -893,7 +892,7 @@ int batch_rebuild(teuk_batch *b) {
     sd.g = s->g;
     sd.d = s->d;
     sd.has_src = s->has_src ? 1 : 0;
-    sd.ngroups = s->P.M / (kRows * b->tiling.wpb);
+    sd.ngroups = stage_groups(s->P.M, b->tiling.wpb);
     sd.nt = member_steps(s);
     sd.tiles = (s->P.N / kWave) * sd.ngroups;
     if (s->k_host >= sd.nt) continue;
@@ -1006,8 +1005,6 @@ int teuk_batch_set_tiling(teuk_batch *b, int waves_per_block, int waves_per_eu, 
   if (!(waves_per_block == 1 || waves_per_block == 2 || waves_per_block == 4)) return TEUK_EINVAL;
   if (!(waves_per_eu == 2 || waves_per_eu == 4) || (order != 0 && order != 1)) return TEUK_EINVAL;
   if (chunk < 1 || chunk > 256) return TEUK_EINVAL;
-  for (teuk_solver *s : b->members)
-    if (s->P.M % (kRows * waves_per_block) != 0) return TEUK_EINVAL;
   b->tiling.wpb = waves_per_block;
   b->tiling.wpe = waves_per_eu;
   b->order = order;
@@ -1019,7 +1016,6 @@ int teuk_batch_set_tiling(teuk_batch *b, int waves_per_block, int waves_per_eu, 
 int teuk_batch_add(teuk_batch *b, teuk_solver *s) {
   if (!b || !s || s->batch) return TEUK_EINVAL;
   if (s->P.integrator != b->integrator) return TEUK_EINVAL;
-  if (s->P.M % (kRows * b->tiling.wpb) != 0) return TEUK_EINVAL;
   const long long bytes = teuk_footprint(&s->P, member_steps(s));
   if (b->budget > 0 && b->used + bytes > b->budget) return TEUK_ENOMEM;
   HIPCHK(hipStreamSynchronize(s->stream));  // set-up work of the solver's own stream

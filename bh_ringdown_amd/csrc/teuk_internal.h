@@ -169,6 +169,9 @@ hipError_t launch_stage_batch(const SolveDesc *descs, const int2 *tiles, int nti
 hipError_t launch_finalize_batch(const SolveDesc *descs, int nsolves, int n, hipStream_t s);
 // Default launch shape for an M-row grid.
 Tiling default_tiling(int M);
+// Blocks per rho strip for wpb waves per block (the last may hold surplus
+// waves, which return at once).
+int stage_groups(int M, int wpb);
 bool source_available();
 
 // Host helpers (teuk_host.cpp).

@@ -632,13 +632,20 @@ hipError_t launch_init_coeffs(const Grid &g, const DevState &d, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Waves per block: the largest of 4, 2, 1 that divides the theta chunks, so
+// no block carries idle waves.
 Tiling default_tiling(int M) {
   const int chunks = M / kRows;
   Tiling t;
-  t.wpb = chunks >= 4 ? 4 : (chunks >= 2 ? 2 : 1);
+  t.wpb = chunks % 4 == 0 ? 4 : (chunks % 2 == 0 ? 2 : 1);
   t.wpe = TEUK_STAGE_WAVES_PER_EU;
   return t;
 }
+
+// Blocks of wpb waves per rho strip; when wpb does not divide the chunks,
+// the surplus waves of the last block find b0 >= M and return at once (the
+// stage kernel has no block-level barrier).
+int stage_groups(int M, int wpb) { return (M / kRows + wpb - 1) / wpb; }
 
 // k_stage<I, S, W> for the runtime (stage, wpe) pair.
 template <int I, int S>
@@ -667,8 +674,8 @@ static hipError_t stage_batch_one(const SolveDesc *descs, const int2 *tiles, int
 
 hipError_t launch_stage(const Grid &g, const DevState &d, int integrator, int stage, bool has_src, int k_rel,
                         Tiling t, hipStream_t s) {
-  if (t.wpb < 1 || t.wpb > 4 || g.M % (kRows * t.wpb) != 0) return hipErrorInvalidValue;
-  const int ngroups = g.M / (kRows * t.wpb);
+  if (t.wpb < 1 || t.wpb > 4 || g.M % kRows != 0) return hipErrorInvalidValue;
+  const int ngroups = stage_groups(g.M, t.wpb);
   const int nb = (g.N / kWave) * ngroups;
   const dim3 grid(nb), block(kWave, t.wpb);
   const int hs = has_src ? 1 : 0;

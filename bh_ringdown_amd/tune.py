@@ -76,8 +76,6 @@ def tune_solver(N, M, integrator=1, steps=60, repeats=2, store=True):
     (best, {candidate: us per stage launch})."""
     times = {}
     for wpb, wpe in itertools.product((1, 2, 4), (2, 4)):
-        if M % (8 * wpb):
-            continue
         p = _probe_params(N, M, integrator, steps * repeats + 10)
         with teuk.Solver(p) as s:
             s.set_tiling(wpb, wpe)
@@ -102,8 +100,6 @@ def tune_batch(shapes, integrator=1, steps=64, store=True, chunks=(16, 32, 64)):
     (best, {candidate: seconds per step})."""
     times = {}
     for wpb, wpe, order, chunk in itertools.product((1, 2, 4), (2, 4), (0, 1), chunks):
-        if any(m % (8 * wpb) for _, m in shapes):
-            continue
         solvers = [teuk.Solver(_probe_params(n, m, integrator, 3 * steps + 10)) for n, m in shapes]
         try:
             with teuk.Batch(integrator=integrator) as b:

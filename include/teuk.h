@@ -160,9 +160,11 @@ int teuk_timing(teuk_solver *s, long long *launches, double *ms);
  * schedule (admission control for batching/serving). */
 long long teuk_footprint(const teuk_params *p, int nsteps);
 
-/* Launch shape of the stage kernels: wavefronts per block (1, 2, 4; M must
- * be a multiple of 8 x that) and the occupancy the kernels were compiled
- * for (2 or 4 waves per SIMD).  Results do not depend on it. */
+/* Launch shape of the stage kernels: wavefronts per block (1, 2, 4; each
+ * wavefront takes 8 theta rows, and when the M/8 row chunks are not a
+ * multiple of it the surplus wavefronts of a strip's last block stay idle)
+ * and the occupancy the kernels were compiled for (2 or 4 waves per SIMD).
+ * Results do not depend on it. */
 int teuk_set_tiling(teuk_solver *s, int waves_per_block, int waves_per_eu);
 
 /* The 12 per-step source coefficients (the canonical T_4 coefficients times

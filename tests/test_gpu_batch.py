@@ -39,11 +39,13 @@ CASES = [  # N, M, xmin, xmax, a, m, steps, seed, source
     (512, 16, -60.0, 90.0, 0.9, -3, 45, 2, False),
     (128, 64, -30.0, 50.0, 0.0, 1, 101, 3, True),
     (384, 32, -50.0, 70.0, 0.7, 0, 64, 4, False),
+    (256, 24, -40.0, 60.0, 0.6, 2, 50, 5, False),   # 3 and 5 theta chunks: partial blocks
+    (128, 40, -30.0, 50.0, 0.8, -2, 38, 6, True),
 ]
 
 
 @pytest.mark.parametrize("integrator", [1, 0])
-@pytest.mark.parametrize("wpb,wpe,order,chunk", [(2, 2, 0, 32), (1, 4, 1, 7)])
+@pytest.mark.parametrize("wpb,wpe,order,chunk", [(2, 2, 0, 32), (1, 4, 1, 7), (4, 2, 0, 16)])
 def test_batch_members_equal_alone(integrator, wpb, wpe, order, chunk):
     specs = [member(N, M, x0, x1, a, m, st, integrator, seed, src) for N, M, x0, x1, a, m, st, seed, src in CASES]
     steps = [c[6] for c in CASES]
@@ -104,8 +106,11 @@ def test_solver_in_batch_refuses_own_stepping():
         s.sync()
 
 
-def test_single_solver_tiling_is_bitwise_neutral():
-    p, u0, rows = member(512, 32, -40.0, 60.0, 0.6, 2, 40, 1, 5, True)
+@pytest.mark.parametrize("M", [32, 40])
+def test_single_solver_tiling_is_bitwise_neutral(M):
+    """Every launch shape gives the same bits, also where the waves per block
+    do not divide the theta chunks (M = 40: 5 chunks)."""
+    p, u0, rows = member(512, M, -40.0, 60.0, 0.6, 2, 40, 1, 5, True)
     ref = run_alone(p, u0, rows, 40)[0]
     _, r_c, th = teuk.grid(p)
     for wpb, wpe in [(1, 2), (2, 4), (4, 4)]:

@@ -120,7 +120,11 @@ def test_plain_1024x32_full_run(integrator):
 
 @pytest.mark.parametrize("N,M,xmin,xmax,cfl", [
     (128, 16, -25.0, 25.0, 0.5), (256, 32, -50.0, 50.0, 0.5),
-    (1024, 64, -50.0, 50.0, 0.5), (512, 128, -50.0, 50.0, 0.25)])
+    (1024, 64, -50.0, 50.0, 0.5), (512, 128, -50.0, 50.0, 0.25),
+    # theta chunk counts 3, 5, 6 (default launch shapes of 1, 1, 2 waves per
+    # block) and the largest M with the smallest N
+    (128, 24, -25.0, 25.0, 0.5), (256, 40, -50.0, 50.0, 0.5), (256, 48, -50.0, 50.0, 0.5),
+    (128, 256, -25.0, 25.0, 0.25)])
 def test_sizes(N, M, xmin, xmax, cfl):
     check(make(N, M, xmin, xmax, 0.5, 2, 60, dt_over_dx=cfl), 60)
 

@@ -8,6 +8,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "teuk.h"
@@ -426,76 +428,17 @@ int create_solver(const teuk_params *p, const double *r_h, const double *r_c, co
   const size_t NM = (size_t)p->N * p->M;
   DevState &d = s->d;
   const int nstates = p->integrator == 0 ? 3 : (p->integrator == 1 ? 4 : 2);
-  double *states = nullptr;
-  if ((rc = dalloc(s, &states, nstates * 4 * NM))) return fail(rc);
-  d.U = states;
-  d.C = states + 4 * NM;   // staggered scheme: the half-point fields
-  d.D = nstates >= 3 ? states + 8 * NM : nullptr;
-  d.A = nstates == 4 ? states + 12 * NM : nullptr;
-  if ((rc = dalloc(s, &d.ktab, kNK * (size_t)p->N))) return fail(rc);
-  if ((rc = dalloc(s, &d.rtab, kNR * (size_t)p->M))) return fail(rc);
-  if ((rc = dalloc(s, &d.dbdx, NM))) return fail(rc);
-  if ((rc = dalloc(s, &d.scratch, NM))) return fail(rc);
-  if ((rc = dalloc(s, &d.theta, p->M))) return fail(rc);
-  if ((rc = dalloc(s, &d.r_c, p->N))) return fail(rc);
-  if ((rc = dalloc(s, &d.patch, (size_t)kRing * 2 * p->ptsx * p->ptsy))) return fail(rc);
-  if ((rc = dalloc(s, &d.patch_org, 2 * kRing))) return fail(rc);
-  double *extw = nullptr, *projw = nullptr;
-  if ((rc = dalloc(s, &d.ext.integ, 4 * (size_t)p->M))) return fail(rc);
-  if ((rc = dalloc(s, &d.ext.snap, (size_t)g.nrec_cap * 6 * p->M))) return fail(rc);
-  if ((rc = dalloc(s, &d.ext.rec, (size_t)g.nrec_cap * 16))) return fail(rc);
-  if ((rc = dalloc(s, &d.ext.rec_k, (size_t)g.nrec_cap * 2))) return fail(rc);
-  if ((rc = dalloc(s, &extw, 2 * (size_t)p->M + 1))) return fail(rc);
   const size_t nproj = 7 * (size_t)(5 * (p->M / 4)) * 3;
-  if ((rc = dalloc(s, &projw, nproj))) return fail(rc);
-  d.ext.extw = extw;
-  d.ext.projw = projw;
-  if ((rc = dalloc(s, &d.ctl, 1))) return fail(rc);
-  if ((rc = dalloc(s, &d.phase, 2 * (size_t)p->N))) return fail(rc);
+
+  // Host-side contents of the tables: source phases, extraction weights
+  // (body.cu:175-183, rr is the extraction radius), coefficient column and
+  // row tables, projection weights, patch origins, the control block.
+  std::vector<double> ph(2 * (size_t)p->N), ph_h;
+  host_source_phase(p->N, p->mass, p->a, p->m, r_c, ph.data());
   if (g.stag) {
-    if ((rc = dalloc(s, &d.ktab_h, kNK * (size_t)p->N))) return fail(rc);
-    if ((rc = dalloc(s, &d.dbdx_h, NM))) return fail(rc);
-    if ((rc = dalloc(s, &d.r_h, p->N))) return fail(rc);
-    if ((rc = dalloc(s, &d.phase_h, 2 * (size_t)p->N))) return fail(rc);
+    ph_h.resize(ph.size());
+    host_source_phase(p->N, p->mass, p->a, p->m, r_h, ph_h.data());
   }
-  d.sched = nullptr;
-  d.nsched = 0;
-  {
-    std::vector<double> ph(2 * (size_t)p->N);
-    host_source_phase(p->N, p->mass, p->a, p->m, r_c, ph.data());
-    if (hipMemcpy(d.phase, ph.data(), ph.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
-      return fail(TEUK_EHIP);
-    if (g.stag) {
-      host_source_phase(p->N, p->mass, p->a, p->m, r_h, ph.data());
-      if (hipMemcpy(d.phase_h, ph.data(), ph.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-          hipMemcpy(d.r_h, r_h, p->N * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
-        return fail(TEUK_EHIP);
-    }
-  }
-
-  // Zero fields and integrals (kern.cu:164-167, body.cu:86-91).
-  if (hipMemsetAsync(states, 0, nstates * 4 * NM * sizeof(double), s->stream) != hipSuccess) return fail(TEUK_EHIP);
-  if (hipMemsetAsync(d.ext.integ, 0, 4 * (size_t)p->M * sizeof(double), s->stream) != hipSuccess)
-    return fail(TEUK_EHIP);
-  if (hipMemsetAsync(d.patch, 0, (size_t)kRing * 2 * p->ptsx * p->ptsy * sizeof(double), s->stream) != hipSuccess)
-    return fail(TEUK_EHIP);
-  Ctl c0{};
-  c0.k = 0;
-  c0.nrec = 0;
-  c0.status = 0;
-  c0.timer = g.start;
-  if (hipMemcpyAsync(d.ctl, &c0, sizeof(Ctl), hipMemcpyHostToDevice, s->stream) != hipSuccess)
-    return fail(TEUK_EHIP);
-  if (hipMemcpyAsync(d.theta, theta, p->M * sizeof(double), hipMemcpyHostToDevice, s->stream) != hipSuccess)
-    return fail(TEUK_EHIP);
-  if (hipMemcpyAsync(d.r_c, r_c, p->N * sizeof(double), hipMemcpyHostToDevice, s->stream) != hipSuccess)
-    return fail(TEUK_EHIP);
-  std::vector<int> org(2 * kRing, -(1 << 29));
-  if (hipMemcpyAsync(d.patch_org, org.data(), org.size() * sizeof(int), hipMemcpyHostToDevice, s->stream) !=
-      hipSuccess)
-    return fail(TEUK_EHIP);
-
-  // Extraction weights, body.cu:175-183 (rr is the extraction radius).
   std::vector<double> h_extw(2 * p->M + 1);
   const double rr = r_c[g.je];
   for (int q = 0; q < p->M; q++) {
@@ -513,21 +456,87 @@ int create_solver(const teuk_params *p, const double *r_h, const double *r_c, co
     rtab_h.resize(rtab.size());
     host_coeff_tables(p->N, p->M, p->xmin, p->xmax, p->dx, p->mass, p->a, p->m, r_h, theta, ktab_h.data(),
                       rtab_h.data(), -0.5, true);
-    if (hipMemcpyAsync(d.ktab_h, ktab_h.data(), ktab_h.size() * sizeof(double), hipMemcpyHostToDevice,
-                       s->stream) != hipSuccess)
-      return fail(TEUK_EHIP);
   }
   std::vector<double> h_projw(nproj);
   host_projection_weights(p->M, p->m, h_projw.data());
-  if (hipMemcpyAsync(d.ktab, ktab.data(), ktab.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) !=
-          hipSuccess ||
-      hipMemcpyAsync(d.rtab, rtab.data(), rtab.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) !=
-          hipSuccess ||
-      hipMemcpyAsync(extw, h_extw.data(), h_extw.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) !=
-          hipSuccess ||
-      hipMemcpyAsync(projw, h_projw.data(), h_projw.size() * sizeof(double), hipMemcpyHostToDevice, s->stream) !=
-          hipSuccess)
-    return fail(TEUK_EHIP);
+  std::vector<int> org(2 * kRing, -(1 << 29));
+  Ctl c0{};
+  c0.k = 0;
+  c0.nrec = 0;
+  c0.status = 0;
+  c0.timer = g.start;
+
+  // All of the solver's device memory is one allocation, carved in 256-byte
+  // aligned pieces: the tables copied from the host first (one copy), then
+  // the memory that starts at zero (one fill), then the rest.
+  size_t bytes = 0;
+  struct Piece {
+    void **dst;
+    size_t off;
+  };
+  std::vector<Piece> pieces;
+  auto carve = [&](auto **dst, size_t count) {
+    using T = std::remove_reference_t<decltype(**dst)>;
+    pieces.push_back({reinterpret_cast<void **>(dst), bytes});
+    const size_t off = bytes;
+    bytes += (std::max<size_t>(count, 1) * sizeof(T) + 255) & ~(size_t)255;
+    return off;
+  };
+  std::vector<std::pair<size_t, std::pair<const void *, size_t>>> init;   // offset, host source, bytes
+  auto carve_init = [&](auto **dst, const auto *src, size_t count) {
+    using T = std::remove_reference_t<decltype(**dst)>;
+    init.push_back({carve(dst, count), {static_cast<const void *>(src), count * sizeof(T)}});
+  };
+  double *extw = nullptr, *projw = nullptr;
+  carve_init(&d.phase, ph.data(), ph.size());
+  carve_init(&d.theta, theta, (size_t)p->M);
+  carve_init(&d.r_c, r_c, (size_t)p->N);
+  carve_init(&d.ktab, ktab.data(), ktab.size());
+  carve_init(&d.rtab, rtab.data(), rtab.size());
+  carve_init(&extw, h_extw.data(), h_extw.size());
+  carve_init(&projw, h_projw.data(), h_projw.size());
+  carve_init(&d.patch_org, org.data(), org.size());
+  carve_init(&d.ctl, &c0, 1);
+  if (g.stag) {
+    carve_init(&d.phase_h, ph_h.data(), ph_h.size());
+    carve_init(&d.r_h, r_h, (size_t)p->N);
+    carve_init(&d.ktab_h, ktab_h.data(), ktab_h.size());
+  }
+  const size_t init_bytes = bytes;
+  double *states = nullptr;
+  // Zero fields and integrals (kern.cu:164-167, body.cu:86-91), empty patches.
+  carve(&states, nstates * 4 * NM);
+  carve(&d.ext.integ, 4 * (size_t)p->M);
+  carve(&d.patch, (size_t)kRing * 2 * p->ptsx * p->ptsy);
+  const size_t zero_end = bytes;
+  carve(&d.dbdx, NM);
+  carve(&d.scratch, NM);
+  carve(&d.ext.snap, (size_t)g.nrec_cap * 6 * p->M);
+  carve(&d.ext.rec, (size_t)g.nrec_cap * 16);
+  carve(&d.ext.rec_k, (size_t)g.nrec_cap * 2);
+  if (g.stag) carve(&d.dbdx_h, NM);
+
+  char *base = nullptr;
+  if ((rc = dalloc(s, &base, bytes))) return fail(rc);
+  for (const Piece &pc : pieces) *pc.dst = base + pc.off;
+  d.U = states;
+  d.C = states + 4 * NM;   // staggered scheme: the half-point fields
+  d.D = nstates >= 3 ? states + 8 * NM : nullptr;
+  d.A = nstates == 4 ? states + 12 * NM : nullptr;
+  d.ext.extw = extw;
+  d.ext.projw = projw;
+  d.sched = nullptr;
+  d.nsched = 0;
+  {
+    std::vector<char> staging(init_bytes, 0);
+    for (const auto &e : init) std::memcpy(staging.data() + e.first, e.second.first, e.second.second);
+    if (hipMemcpyAsync(base, staging.data(), init_bytes, hipMemcpyHostToDevice, s->stream) != hipSuccess)
+      return fail(TEUK_EHIP);
+    if (hipMemsetAsync(base + init_bytes, 0, zero_end - init_bytes, s->stream) != hipSuccess)
+      return fail(TEUK_EHIP);
+    // the staging buffer is pageable: the copy has completed when the call returns
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(TEUK_EHIP);
+  }
   if ((g.stag ? launch_init_so2(g, d, s->stream) : launch_init_coeffs(g, d, s->stream)) != hipSuccess)
     return fail(TEUK_EHIP);
   if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(TEUK_EHIP);
@@ -604,13 +613,21 @@ int teuk_set_source(teuk_solver *s, const teuk_source_step *steps, int nsteps) {
     hdev[k] = prep[k].dev;
   }
   HIPCHK(hipStreamSynchronize(s->stream));
-  SrcDev *dst = nullptr;
-  teuk_t4::StepIn *din = nullptr;
-  int rc = dalloc(s, &dst, (size_t)nsteps);
+  // The schedule and the particle states it is computed from, in one
+  // allocation and one copy.
+  const size_t b_dev = ((size_t)nsteps * sizeof(SrcDev) + 255) & ~(size_t)255;
+  const size_t b_in = (size_t)nsteps * sizeof(teuk_t4::StepIn);
+  char *blk = nullptr;
+  int rc = dalloc(s, &blk, b_dev + b_in);
   if (rc) return rc;
-  if ((rc = dalloc(s, &din, (size_t)nsteps))) return rc;
-  HIPCHK(hipMemcpy(dst, hdev.data(), (size_t)nsteps * sizeof(SrcDev), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(din, hin.data(), (size_t)nsteps * sizeof(teuk_t4::StepIn), hipMemcpyHostToDevice));
+  SrcDev *dst = reinterpret_cast<SrcDev *>(blk);
+  teuk_t4::StepIn *din = reinterpret_cast<teuk_t4::StepIn *>(blk + b_dev);
+  {
+    std::vector<char> staging(b_dev + b_in, 0);
+    std::memcpy(staging.data(), hdev.data(), (size_t)nsteps * sizeof(SrcDev));
+    std::memcpy(staging.data() + b_dev, hin.data(), b_in);
+    HIPCHK(hipMemcpy(blk, staging.data(), b_dev + b_in, hipMemcpyHostToDevice));
+  }
   HIPCHK(launch_source_coeffs(din, dst, nsteps, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   s->d.sched = dst;

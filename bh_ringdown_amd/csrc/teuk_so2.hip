@@ -16,9 +16,11 @@
 //                  (Old/body.cu:199-216), the edges through LDS.
 // Each thread owns one point; a wavefront covers 64 consecutive rho columns
 // of one theta row, so every access is coalesced; the neighbour reads (one
-// column to the side, one row up and down) are cache hits.  The coefficient
-// fields are rebuilt per point from the column constants of their point set
-// and the row values, as in the high-order stage kernel.
+// column to the side, one row up and down) are cache hits.  A complex value
+// (re, im) is one 16-byte access, addressed by a 32-bit byte offset from the
+// plane base.  The coefficient fields are rebuilt per point from the column
+// constants of their point set, staged once per block in LDS, and the row
+// values, as in the high-order stage kernel.
 //
 // All arithmetic goes through teuk_point.h, built without contraction, so the
 // results equal the oracle's bit for bit.
@@ -43,6 +45,23 @@ __device__ __forceinline__ Coef so2_coef(const double *__restrict__ ktab, int N,
 #pragma unroll
   for (int i = 0; i < kNK; i++) K[i] = ktab[(size_t)i * N + a];
   return teuk_pt::coef_point(K, sn2, cs, Y, [](double x) { return __dsqrt_rn(x); });
+}
+
+// Complex value of plane h (0: Psi, 1: Pi) at point c: byte h cb + 16 c.
+__device__ __forceinline__ double2 cld(const double *base, unsigned off) {
+  return *reinterpret_cast<const double2 *>(reinterpret_cast<const char *>(base) + off);
+}
+__device__ __forceinline__ void cst(double *base, unsigned off, double re, double im) {
+  *reinterpret_cast<double2 *>(reinterpret_cast<char *>(base) + off) = make_double2(re, im);
+}
+
+// The column constants of the block's 64 columns into LDS (Kl[i][tx]); every
+// thread of the block takes part, then the block waits.
+__device__ __forceinline__ void so2_stage_K(const double *__restrict__ ktab, int N, int a0,
+                                            double (&Kl)[kNK][kWave]) {
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  for (int i = ty; i < kNK; i += kSo2Rows) Kl[i][tx] = ktab[(size_t)i * N + a0 + tx];
+  __syncthreads();
 }
 
 // bb on rows 1 <= k < M-1 (Old/kern.cu:90 and :143, bb at :123 and :176) of
@@ -89,13 +108,16 @@ __device__ __forceinline__ void so2_source(const Grid &g, const DevState &d, int
 // Right-hand side at one point of either set (Old/kern.cu:214-284,
 // :365-434): pv = the averaged fields at rows b-1, b, b+1 (q) and b (p),
 // dxv = the four rho differences.
-__device__ __forceinline__ void so2_rhs_point(const Grid &g, const DevState &d, const double *__restrict__ ktab,
+__device__ __forceinline__ void so2_rhs_point(const Grid &g, const DevState &d, const double (&Kl)[kNK][kWave],
                                               const double *__restrict__ dbdx, int b, int a, const double qr[3],
                                               const double qi[3], double pr, double pi, const double dxv[4],
                                               double sr, double si, double r[4]) {
   const double *__restrict__ rtab = d.rtab;
   const int M = g.M;
-  const Coef cf = so2_coef(ktab, g.N, a, rtab[b], rtab[M + b], rtab[2 * M + b]);
+  double K[kNK];
+#pragma unroll
+  for (int i = 0; i < kNK; i++) K[i] = Kl[i][threadIdx.x];
+  const Coef cf = teuk_pt::coef_point(K, rtab[b], rtab[M + b], rtab[2 * M + b], [](double x) { return __dsqrt_rn(x); });
   const double dre = teuk_pt::fma_(-dbdx[(size_t)b * g.N + a], cf.ee, cf.dre);
   const double ct = rtab[3 * M + b];   // cos/sin (Old/kern.cu:241-243)
   teuk_pt::Pt x;
@@ -118,34 +140,43 @@ __device__ __forceinline__ void so2_rhs_point(const Grid &g, const DevState &d, 
 // Half step: X_h = avg(X[a], X[a-1]) + dt/2 L_h, rows 1..M-2, 1 <= a < N
 // (Old/kern.cu:196-303).  Reads X (d.U) only, writes X_h (d.C).
 __global__ __launch_bounds__(256) void k_so2_half(Grid g, DevState d, int has_src, int k_rel) {
-  const int a = blockIdx.x * kWave + threadIdx.x;
-  const int b = blockIdx.y * kSo2Rows + threadIdx.y;
+  __shared__ double Kl[kNK][kWave];
+  const int a0 = blockIdx.x * kWave;
+  const int a = a0 + threadIdx.x;
+  // the row is uniform over the wavefront: row tables become scalar loads
+  const int b = blockIdx.y * kSo2Rows + __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int N = g.N, M = g.M;
+  so2_stage_K(d.ktab_h, N, a0, Kl);
   if (a < 1 || a >= N || b < 1 || b >= M - 1) return;
-  const size_t NM = (size_t)N * M;
+  const unsigned cb = 16u * (unsigned)N * (unsigned)M;
   const double *__restrict__ X = d.U;
   double *__restrict__ Xh = d.C;
-  const size_t c = (size_t)b * N + a;
+  const unsigned c = 16u * (unsigned)(b * N + a);
+  const unsigned rowb = 16u * (unsigned)N;
   double qr[3], qi[3];
 #pragma unroll
   for (int i = 0; i < 3; i++) {
-    const size_t ci = c + (size_t)(i - 1) * N;
-    qr[i] = teuk_pt::so2_avg(X[fidx(0, ci, NM)], X[fidx(0, ci - 1, NM)]);
-    qi[i] = teuk_pt::so2_avg(X[fidx(1, ci, NM)], X[fidx(1, ci - 1, NM)]);
+    const unsigned ci = c + rowb * i - rowb;
+    const double2 x1 = cld(X, ci), x0 = cld(X, ci - 16u);
+    qr[i] = teuk_pt::so2_avg(x1.x, x0.x);
+    qi[i] = teuk_pt::so2_avg(x1.y, x0.y);
   }
-  const double pr = teuk_pt::so2_avg(X[fidx(2, c, NM)], X[fidx(2, c - 1, NM)]);
-  const double pi = teuk_pt::so2_avg(X[fidx(3, c, NM)], X[fidx(3, c - 1, NM)]);
+  const double2 p1 = cld(X, cb + c), p0 = cld(X, cb + c - 16u);
+  const double2 q1 = cld(X, c), q0 = cld(X, c - 16u);
+  const double pr = teuk_pt::so2_avg(p1.x, p0.x);
+  const double pi = teuk_pt::so2_avg(p1.y, p0.y);
   double dxv[4];
-#pragma unroll
-  for (int f = 0; f < 4; f++) dxv[f] = teuk_pt::so2_dx(X[fidx(f, c, NM)], X[fidx(f, c - 1, NM)], g.dx);
+  dxv[0] = teuk_pt::so2_dx(q1.x, q0.x, g.dx);
+  dxv[1] = teuk_pt::so2_dx(q1.y, q0.y, g.dx);
+  dxv[2] = teuk_pt::so2_dx(p1.x, p0.x, g.dx);
+  dxv[3] = teuk_pt::so2_dx(p1.y, p0.y, g.dx);
   double sr = 0.0, si = 0.0;
   if (has_src) so2_source(g, d, 2 * (d.ctl->k + k_rel), b, a, sr, si);
   double r[4];
-  so2_rhs_point(g, d, d.ktab_h, d.dbdx_h, b, a, qr, qi, pr, pi, dxv, sr, si, r);
+  so2_rhs_point(g, d, Kl, d.dbdx_h, b, a, qr, qi, pr, pi, dxv, sr, si, r);
   const double h = 0.5 * g.dt;
-  const double v0[4] = {qr[1], qi[1], pr, pi};
-#pragma unroll
-  for (int f = 0; f < 4; f++) Xh[fidx(f, c, NM)] = teuk_pt::axpy(h, r[f], v0[f]);
+  cst(Xh, c, teuk_pt::axpy(h, r[0], qr[1]), teuk_pt::axpy(h, r[1], qi[1]));
+  cst(Xh, cb + c, teuk_pt::axpy(h, r[2], pr), teuk_pt::axpy(h, r[3], pi));
 }
 
 // Full step, edges and extraction in one pass (Old/kern.cu:305-522 and
@@ -164,44 +195,73 @@ __global__ __launch_bounds__(256) void k_so2_half(Grid g, DevState d, int has_sr
 //     from the final value and snapshots them every Fr steps.
 __global__ __launch_bounds__(256) void k_so2_full(Grid g, DevState d, int has_src, int k_rel) {
   __shared__ double tile[4][kSo2Rows][kWave];
+  __shared__ double Kl[kNK][kWave];
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const int a = blockIdx.x * kWave + tx;
-  const int b = blockIdx.y * kSo2Rows + ty;
+  const int a0 = blockIdx.x * kWave;
+  const int a = a0 + tx;
+  // the row is uniform over the wavefront: row tables become scalar loads
+  const int b = blockIdx.y * kSo2Rows + __builtin_amdgcn_readfirstlane(ty);
   const int N = g.N, M = g.M;
   const size_t NM = (size_t)N * M;
+  const unsigned cb = 16u * (unsigned)N * (unsigned)M;
   const double *__restrict__ Xh = d.C;
   double *__restrict__ X = d.U;
   const size_t c = (size_t)b * N + a;
+  const unsigned cB = 16u * (unsigned)c;
+  so2_stage_K(d.ktab, N, a0, Kl);
   const bool inner = a >= 1 && a < N - 1 && b >= 1 && b < M - 1;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   if (inner) {
-    // X_h at row rr, column col, with the theta edge rows of kernel_boundary1
-    auto xh = [&](int f, int rr, int col) -> double {
-      auto at = [&](int row) { return Xh[fidx(f, (size_t)row * N + col, NM)]; };
-      if (rr == 0) return g.even_m ? teuk_pt::so2_edge(at(1), at(2)) : 0.0;
-      if (rr == M - 1) return g.even_m ? teuk_pt::so2_edge(at(M - 2), at(M - 3)) : 0.0;
-      return at(rr);
+    // Psi_h (re, im) at row rr, columns a and a+1, with the theta edge rows
+    // of kernel_boundary1 formed from the next two rows
+    auto xh = [&](int rr, double2 &c0, double2 &c1) {
+      auto at = [&](int row, double2 &v0, double2 &v1) {
+        const unsigned o = 16u * (unsigned)(row * N + a);
+        v0 = cld(Xh, o);
+        v1 = cld(Xh, o + 16u);
+      };
+      if (rr == 0 || rr == M - 1) {
+        double2 e1a, e1b, e2a, e2b;
+        at(rr == 0 ? 1 : M - 2, e1a, e1b);
+        at(rr == 0 ? 2 : M - 3, e2a, e2b);
+        if (g.even_m) {
+          c0 = make_double2(teuk_pt::so2_edge(e1a.x, e2a.x), teuk_pt::so2_edge(e1a.y, e2a.y));
+          c1 = make_double2(teuk_pt::so2_edge(e1b.x, e2b.x), teuk_pt::so2_edge(e1b.y, e2b.y));
+        } else {
+          c0 = c1 = make_double2(0.0, 0.0);
+        }
+        return;
+      }
+      at(rr, c0, c1);
     };
     double qr[3], qi[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-      qr[i] = teuk_pt::so2_avg(xh(0, b + i - 1, a), xh(0, b + i - 1, a + 1));
-      qi[i] = teuk_pt::so2_avg(xh(1, b + i - 1, a), xh(1, b + i - 1, a + 1));
+      double2 x0, x1;
+      xh(b + i - 1, x0, x1);
+      qr[i] = teuk_pt::so2_avg(x0.x, x1.x);
+      qi[i] = teuk_pt::so2_avg(x0.y, x1.y);
     }
-    const double pr = teuk_pt::so2_avg(Xh[fidx(2, c, NM)], Xh[fidx(2, c + 1, NM)]);
-    const double pi = teuk_pt::so2_avg(Xh[fidx(3, c, NM)], Xh[fidx(3, c + 1, NM)]);
+    const double2 q0 = cld(Xh, cB), q1 = cld(Xh, cB + 16u);
+    const double2 p0 = cld(Xh, cb + cB), p1 = cld(Xh, cb + cB + 16u);
+    const double pr = teuk_pt::so2_avg(p0.x, p1.x);
+    const double pi = teuk_pt::so2_avg(p0.y, p1.y);
     double dxv[4];
-#pragma unroll
-    for (int f = 0; f < 4; f++) dxv[f] = teuk_pt::so2_dx(Xh[fidx(f, c + 1, NM)], Xh[fidx(f, c, NM)], g.dx);
+    dxv[0] = teuk_pt::so2_dx(q1.x, q0.x, g.dx);
+    dxv[1] = teuk_pt::so2_dx(q1.y, q0.y, g.dx);
+    dxv[2] = teuk_pt::so2_dx(p1.x, p0.x, g.dx);
+    dxv[3] = teuk_pt::so2_dx(p1.y, p0.y, g.dx);
     double sr = 0.0, si = 0.0;
     if (has_src) so2_source(g, d, 2 * (d.ctl->k + k_rel) + 1, b, a, sr, si);
     double r[4];
-    so2_rhs_point(g, d, d.ktab, d.dbdx, b, a, qr, qi, pr, pi, dxv, sr, si, r);
-#pragma unroll
-    for (int f = 0; f < 4; f++) {
-      v[f] = teuk_pt::axpy(g.dt, r[f], X[fidx(f, c, NM)]);
-      X[fidx(f, c, NM)] = v[f];
-    }
+    so2_rhs_point(g, d, Kl, d.dbdx, b, a, qr, qi, pr, pi, dxv, sr, si, r);
+    const double2 u0 = cld(X, cB), u1 = cld(X, cb + cB);
+    v[0] = teuk_pt::axpy(g.dt, r[0], u0.x);
+    v[1] = teuk_pt::axpy(g.dt, r[1], u0.y);
+    v[2] = teuk_pt::axpy(g.dt, r[2], u1.x);
+    v[3] = teuk_pt::axpy(g.dt, r[3], u1.y);
+    cst(X, cB, v[0], v[1]);
+    cst(X, cb + cB, v[2], v[3]);
   }
 #pragma unroll
   for (int f = 0; f < 4; f++) tile[f][ty][tx] = v[f];

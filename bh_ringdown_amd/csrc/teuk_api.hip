@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <type_traits>
 #include <utility>
@@ -470,15 +471,11 @@ int create_solver(const teuk_params *p, const double *r_h, const double *r_c, co
   // aligned pieces: the tables copied from the host first (one copy), then
   // the memory that starts at zero (one fill), then the rest.
   size_t bytes = 0;
-  struct Piece {
-    void **dst;
-    size_t off;
-  };
-  std::vector<Piece> pieces;
+  std::vector<std::function<void(char *)>> bind;   // point each piece into the allocation
   auto carve = [&](auto **dst, size_t count) {
     using T = std::remove_reference_t<decltype(**dst)>;
-    pieces.push_back({reinterpret_cast<void **>(dst), bytes});
     const size_t off = bytes;
+    bind.push_back([dst, off](char *b) { *dst = reinterpret_cast<T *>(b + off); });
     bytes += (std::max<size_t>(count, 1) * sizeof(T) + 255) & ~(size_t)255;
     return off;
   };
@@ -518,7 +515,7 @@ int create_solver(const teuk_params *p, const double *r_h, const double *r_c, co
 
   char *base = nullptr;
   if ((rc = dalloc(s, &base, bytes))) return fail(rc);
-  for (const Piece &pc : pieces) *pc.dst = base + pc.off;
+  for (const auto &f : bind) f(base);
   d.U = states;
   d.C = states + 4 * NM;   // staggered scheme: the half-point fields
   d.D = nstates >= 3 ? states + 8 * NM : nullptr;

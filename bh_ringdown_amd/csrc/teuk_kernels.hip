@@ -558,13 +558,27 @@ __device__ __forceinline__ void finalize_solve(const Grid &g, const Ext &ext, Ct
     const int r = r_lo + task / 15, part = task % 15;
     const double *sn = ext.snap + (size_t)r * 6 * M;
     double *rec = ext.rec + (size_t)r * 16;
+    // The sums run in the reference's order; their inputs are loaded a group
+    // at a time (M is a multiple of 8, so groups of 8 rows and of 5 Boole
+    // terms divide the loops) so the loads overlap instead of each waiting
+    // out the memory latency in turn.
     if (part == 14) {
       double dedt = 0.0;
-      for (int q = 0; q < M; q++) {
-        const double ir = sn[2 * M + q], ii = sn[3 * M + q];
-        const double psi2 = __dadd_rn(__dmul_rn(ir, ir), __dmul_rn(ii, ii));
-        dedt = __dadd_rn(dedt, __ddiv_rn(__dmul_rn(__dmul_rn(psi2, ext.extw[q]), ext.extw[2 * M]),
-                                         ext.extw[M + q]));
+      const double r6 = ext.extw[2 * M];
+      for (int q0 = 0; q0 < M; q0 += 8) {
+        double ir[8], ii[8], w0[8], w1[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          ir[j] = sn[2 * M + q0 + j];
+          ii[j] = sn[3 * M + q0 + j];
+          w0[j] = ext.extw[q0 + j];
+          w1[j] = ext.extw[M + q0 + j];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          const double psi2 = __dadd_rn(__dmul_rn(ir[j], ir[j]), __dmul_rn(ii[j], ii[j]));
+          dedt = __dadd_rn(dedt, __ddiv_rn(__dmul_rn(__dmul_rn(psi2, w0[j]), r6), w1[j]));
+        }
       }
       rec[1] = dedt;
     } else {
@@ -572,12 +586,21 @@ __device__ __forceinline__ void finalize_solve(const Grid &g, const Ext &ext, Ct
       const double *wl = ext.projw + (size_t)l * nterm * 3;
       const double *dq = sn + (4 + im) * M;
       double h = 0.0;
-      for (int tm = 0; tm < nterm; tm++) {
-        const int pt = 4 * (tm / 5) + (tm % 5);
-        const double v = dq[pt < M ? pt : M - 1];
-        const double term =
-            __dmul_rn(__dmul_rn(__dmul_rn(__dmul_rn(wl[3 * tm], v), wl[3 * tm + 1]), g.dtheta), wl[3 * tm + 2]);
-        h = __dadd_rn(h, term);
+      for (int g5 = 0; g5 < nterm / 5; g5++) {   // Boole group g5: rows 4 g5 .. 4 g5 + 4
+        double v[5], w[15];
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+          const int pt = 4 * g5 + j;
+          v[j] = dq[pt < M ? pt : M - 1];
+        }
+#pragma unroll
+        for (int j = 0; j < 15; j++) w[j] = wl[15 * g5 + j];
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+          const double term =
+              __dmul_rn(__dmul_rn(__dmul_rn(__dmul_rn(w[3 * j], v[j]), w[3 * j + 1]), g.dtheta), w[3 * j + 2]);
+          h = __dadd_rn(h, term);
+        }
       }
       h = __ddiv_rn(__dmul_rn(2.0, h), 45.0);
       rec[2 + part] = im ? -h : h;

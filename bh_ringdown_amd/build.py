@@ -36,10 +36,18 @@ def _run(cmd):
 
 
 def build_lib():
+    """Each source compiled on its own (in parallel), then one link."""
+    from concurrent.futures import ThreadPoolExecutor
     os.makedirs(os.path.join(PKG, "lib"), exist_ok=True)
+    objdir = os.path.join(PKG, "lib", "obj")
+    os.makedirs(objdir, exist_ok=True)
     out = os.path.join(PKG, "lib", "libteuk.so")
     tmp = out + ".tmp"
-    _run([HIPCC, *HIPFLAGS, "-shared", "-o", tmp, *[os.path.join(CSRC, s) for s in LIB_SOURCES]])
+    objs = [os.path.join(objdir, os.path.splitext(src)[0] + ".o") for src in LIB_SOURCES]
+    with ThreadPoolExecutor(max_workers=min(len(LIB_SOURCES), os.cpu_count() or 1)) as ex:
+        list(ex.map(lambda so: _run([HIPCC, *HIPFLAGS, "-c", "-o", so[1], os.path.join(CSRC, so[0])]),
+                    zip(LIB_SOURCES, objs)))
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, out)
     return out
 

@@ -40,6 +40,7 @@ struct teuk_solver {
   long long timed_launches = 0;
   double timed_ms = 0.0;
   std::vector<void *> allocs;
+  void *sched_blk = nullptr;   // the source schedule's allocation (replaced by each teuk_set_source)
 };
 
 // A set of solvers stepped together by the batched kernels (one launch per
@@ -173,6 +174,7 @@ int build_graph(teuk_solver *s, int steps) {
   HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
   int rc = TEUK_OK;
   for (int i = 0; i < steps && rc == TEUK_OK; i++) rc = enqueue_step(s, i);
+  if (rc == TEUK_OK && launch_sentinel(s->g, s->d, s->stream) != hipSuccess) rc = TEUK_EHIP;
   if (rc == TEUK_OK && launch_finalize(s->g, s->d, steps, s->stream) != hipSuccess) rc = TEUK_EHIP;
   hipError_t e = hipStreamEndCapture(s->stream, &graph);
   if (rc != TEUK_OK) {
@@ -552,6 +554,7 @@ void teuk_destroy(teuk_solver *s) {
   if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
   for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
   for (void *q : s->allocs) (void)hipFree(q);
+  if (s->sched_blk) (void)hipFree(s->sched_blk);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -591,6 +594,17 @@ int teuk_set_source(teuk_solver *s, const teuk_source_step *steps, int nsteps) {
   if (!s || nsteps < 0) return TEUK_EINVAL;
   if (s->batch) return TEUK_EINVAL;  // the batch's descriptors hold the schedule
   if (!steps || nsteps == 0) {
+    // A captured graph holds the source launches and the source-on stage
+    // kernels: drop it with the schedule.
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (s->gexec) {
+      (void)hipGraphExecDestroy(s->gexec);
+      s->gexec = nullptr;
+    }
+    if (s->sched_blk) (void)hipFree(s->sched_blk);
+    s->sched_blk = nullptr;
+    s->d.sched = nullptr;
+    s->d.nsched = 0;
     s->has_src = false;
     return TEUK_OK;
   }
@@ -615,18 +629,23 @@ int teuk_set_source(teuk_solver *s, const teuk_source_step *steps, int nsteps) {
   const size_t b_dev = ((size_t)nsteps * sizeof(SrcDev) + 255) & ~(size_t)255;
   const size_t b_in = (size_t)nsteps * sizeof(teuk_t4::StepIn);
   char *blk = nullptr;
-  int rc = dalloc(s, &blk, b_dev + b_in);
-  if (rc) return rc;
+  if (hipMalloc(reinterpret_cast<void **>(&blk), b_dev + b_in) != hipSuccess) return TEUK_ENOMEM;
   SrcDev *dst = reinterpret_cast<SrcDev *>(blk);
   teuk_t4::StepIn *din = reinterpret_cast<teuk_t4::StepIn *>(blk + b_dev);
   {
     std::vector<char> staging(b_dev + b_in, 0);
     std::memcpy(staging.data(), hdev.data(), (size_t)nsteps * sizeof(SrcDev));
     std::memcpy(staging.data() + b_dev, hin.data(), b_in);
-    HIPCHK(hipMemcpy(blk, staging.data(), b_dev + b_in, hipMemcpyHostToDevice));
+    if (hipMemcpy(blk, staging.data(), b_dev + b_in, hipMemcpyHostToDevice) != hipSuccess ||
+        launch_source_coeffs(din, dst, nsteps, s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess) {
+      (void)hipFree(blk);
+      return TEUK_EHIP;
+    }
   }
-  HIPCHK(launch_source_coeffs(din, dst, nsteps, s->stream));
-  HIPCHK(hipStreamSynchronize(s->stream));
+  // the stream is idle: the previous schedule is no longer read
+  if (s->sched_blk) (void)hipFree(s->sched_blk);
+  s->sched_blk = blk;
   s->d.sched = dst;
   s->d.nsched = nsteps;
   s->has_src = true;
@@ -676,6 +695,7 @@ int teuk_step(teuk_solver *s, int nsteps) {
   auto direct = [&](int n) {
     int r = TEUK_OK;
     for (int i = 0; i < n && r == TEUK_OK; i++) r = enqueue_step(s, i);
+    if (r == TEUK_OK && n > 0 && launch_sentinel(s->g, s->d, s->stream) != hipSuccess) r = TEUK_EHIP;
     if (r == TEUK_OK && n > 0 && launch_finalize(s->g, s->d, n, s->stream) != hipSuccess) r = TEUK_EHIP;
     return r;
   };
@@ -701,6 +721,14 @@ int teuk_sync(teuk_solver *s) {
   if (!s) return TEUK_EINVAL;
   HIPCHK(hipStreamSynchronize(run_stream(s)));
   if (s->timing) flush_timing(s);
+  Ctl c;
+  HIPCHK(hipMemcpy(&c, s->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+  return c.status ? TEUK_EDIVERGED : TEUK_OK;
+}
+
+int teuk_status(teuk_solver *s) {
+  if (!s) return TEUK_EINVAL;
+  HIPCHK(hipStreamSynchronize(run_stream(s)));
   Ctl c;
   HIPCHK(hipMemcpy(&c, s->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
   return c.status ? TEUK_EDIVERGED : TEUK_OK;
@@ -964,6 +992,7 @@ int batch_enqueue(teuk_batch *b, int nsteps) {
     int rc = batch_enqueue_step(b, i);
     if (rc) return rc;
   }
+  HIPCHK(launch_sentinel_batch(b->d_desc, (int)b->members.size(), b->stream));
   HIPCHK(launch_finalize_batch(b->d_desc, (int)b->members.size(), nsteps, b->stream));
   return TEUK_OK;
 }

@@ -31,6 +31,9 @@ constexpr int kHalo = 4;            // KO reach in rho (kern.cu:247-256)
 // ring of that many slots (step k uses slot k % kRing).
 constexpr int kRing = 32;
 constexpr int kLdsCols = kWave + 2 * kHalo;
+// Whole-field sentinel: once per chunk of steps every field value is checked;
+// a non-finite value, or a magnitude above this bound, marks the solve failed.
+constexpr double kGrowthLimit = 1e60;
 
 // Per-column constants of the coefficient fields (teuk_point.h).
 using teuk_pt::kK1;
@@ -68,7 +71,8 @@ struct Grid {
 struct Ctl {
   int k;               // completed steps
   int nrec;            // output records written
-  int status;          // 0 ok, 1 non-finite field seen
+  int status;          // 0 ok; bit 0: non-finite Psi at the extraction column,
+                       // bit 1: the whole-field sentinel fired (see kGrowthLimit)
   int pad;
   double timer;        // solver time, advanced by repeated addition (body.cu:214)
 };
@@ -156,6 +160,9 @@ hipError_t launch_source_at(const Grid &g, const DevState &d, int kstep, hipStre
 hipError_t launch_stage(const Grid &g, const DevState &d, int integrator, int stage,
                         bool has_src, int k_rel, Tiling t, hipStream_t s);
 hipError_t launch_finalize(const Grid &g, const DevState &d, int n, hipStream_t s);
+// The whole-field sentinel over the current fields (state U); enqueued once
+// per chunk of steps, before the chunk's launch_finalize.
+hipError_t launch_sentinel(const Grid &g, const DevState &d, hipStream_t s);
 // Staggered second-order scheme (teuk_so2.hip): bb and dbdx of both point
 // sets, and one step (half step, its theta edges, full step, edges and
 // extraction).
@@ -167,6 +174,8 @@ hipError_t launch_source_batch(const SolveDesc *descs, int nsolves, int max_cell
 hipError_t launch_stage_batch(const SolveDesc *descs, const int2 *tiles, int ntiles, int integrator,
                               int stage, int k_rel, Tiling t, hipStream_t s);
 hipError_t launch_finalize_batch(const SolveDesc *descs, int nsolves, int n, hipStream_t s);
+// The sentinel of every member still running before the chunk's finalize.
+hipError_t launch_sentinel_batch(const SolveDesc *descs, int nsolves, hipStream_t s);
 // Default launch shape for an M-row grid.
 Tiling default_tiling(int M);
 // Blocks per rho strip for wpb waves per block (the last may hold surplus

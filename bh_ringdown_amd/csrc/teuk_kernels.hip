@@ -635,6 +635,36 @@ __global__ __launch_bounds__(256) void k_finalize_batch(const SolveDesc *__restr
   finalize_solve(sd.g, sd.d.ext, sd.d.ctl, n_eff);
 }
 
+// Whole-field sentinel (no counterpart in the reference, which checks
+// nothing): every value of the current fields, as (re, im) pairs, against
+// non-finite values and magnitudes above kGrowthLimit; a hit sets status
+// bit 1 of the solve.  `!(|x| <= L)` is also true for NaN.  Grid-stride over
+// the 2 N M pairs of the state.
+__device__ __forceinline__ void sentinel_scan(const double2 *__restrict__ u, size_t n2, Ctl *ctl, size_t i0,
+                                              size_t stride) {
+  bool bad = false;
+  for (size_t i = i0; i < n2; i += stride) {
+    const double2 v = u[i];
+    bad = bad || !(fabs(v.x) <= kGrowthLimit) || !(fabs(v.y) <= kGrowthLimit);
+  }
+  if (bad) atomicOr(&ctl->status, 2);
+}
+
+__global__ __launch_bounds__(256) void k_sentinel(const double2 *__restrict__ u, size_t n2, Ctl *ctl) {
+  sentinel_scan(u, n2, ctl, (size_t)blockIdx.x * blockDim.x + threadIdx.x, (size_t)gridDim.x * blockDim.x);
+}
+
+// blockIdx.y = member; a member that had finished before this chunk (its
+// step counter, which the chunk's finalize advances afterwards, is at its
+// scheduled count) is skipped.
+__global__ __launch_bounds__(256) void k_sentinel_batch(const SolveDesc *__restrict__ descs) {
+  const SolveDesc &sd = descs[blockIdx.y];
+  if (sd.d.ctl->k >= sd.nt) return;
+  const size_t n2 = 2 * (size_t)sd.g.N * sd.g.M;
+  sentinel_scan(reinterpret_cast<const double2 *>(sd.d.U), n2, sd.d.ctl,
+                (size_t)blockIdx.x * blockDim.x + threadIdx.x, (size_t)gridDim.x * blockDim.x);
+}
+
 // ---------------------------------------------------------------------------
 // Particle source on the compact ptsx x ptsy patch (kern.cu:770-2683).
 #include "teuk_source_dev.inc"
@@ -757,6 +787,25 @@ hipError_t launch_source_batch(const SolveDesc *descs, int nsolves, int max_cell
 
 hipError_t launch_finalize(const Grid &g, const DevState &d, int n, hipStream_t s) {
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, g, d.ext, d.ctl, n);
+  return hipGetLastError();
+}
+
+// Blocks of 256 threads, about four pairs per thread, at most 1024 blocks.
+static unsigned sentinel_blocks(size_t n2) {
+  const size_t nb = (n2 + 1023) / 1024;
+  return (unsigned)(nb < 1 ? 1 : (nb > 1024 ? 1024 : nb));
+}
+
+hipError_t launch_sentinel(const Grid &g, const DevState &d, hipStream_t s) {
+  const size_t n2 = 2 * (size_t)g.N * g.M;
+  hipLaunchKernelGGL(k_sentinel, dim3(sentinel_blocks(n2)), dim3(256), 0, s,
+                     reinterpret_cast<const double2 *>(d.U), n2, d.ctl);
+  return hipGetLastError();
+}
+
+hipError_t launch_sentinel_batch(const SolveDesc *descs, int nsolves, hipStream_t s) {
+  if (nsolves <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sentinel_batch, dim3(64, nsolves), dim3(256), 0, s, descs);
   return hipGetLastError();
 }
 

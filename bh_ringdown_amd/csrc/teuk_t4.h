@@ -19,6 +19,20 @@
 // third time derivatives of r and theta zero (body.cu:124), and the azimuthal
 // mode delta(phi - phi_p) -> e^{-i m phi_p}/pi, d_phi -> i m (kern.cu:1270).
 //
+// Normalisation: the reference's generated T_4 is HALF of that expression.
+// Its delta''(r) delta(theta) coefficient for r' = theta' = 0 (kern.cu:2630
+// t7 * t3198 / 2, with t3198 = -t2717 = t26 t2479 / 2 (kern.cu:2552, 2363,
+// 2281), t2479 = t7 t2444 / 4 (:2280), t2444 = -t590 t147 t900 (:2262),
+// t900 = t224 e^{-i m phi_p} / (pi sin), t224 = t51^2 / t38 (:1688, :1368))
+// is -Delta^2 mu (mbar.u)^2 e^{-i m phi_p} / (8 pi Sigma^2 Sigma u^t sin),
+// where -D^2 T_mbmb above gives twice that; and the reference's own waveform
+// for its plunge example (QNM_Extraction/Examples/hm2_a0.7_thi000_thf090.0_n
+// .dat) is half of what the undivided source produces on the same grid and
+// trajectory, for every l and every time (ratio 1.98 at N = 16000, M = 256,
+// 2.00 in the continuum limit).  So the coefficients carry a factor 1/2 with
+// the 1/pi: a circular orbit then radiates half the quadrupole-formula
+// amplitude, |h_22| -> 4 sqrt(pi/5) mu v^2.
+//
 // The result is reduced to the canonical form
 //   T_4 = sum_{i+j<=2} c_ij delta^(i)(r - r_p) delta^(j)(theta - theta_p)
 // with coefficients on the worldline, the form the reference evaluates with
@@ -232,7 +246,8 @@ struct StepIn {
   double cmphi, smphi;    // cos, sin of m phi_p
 };
 
-// Canonical coefficients times e^{-i m phi_p}/pi: out[2*t], out[2*t+1] for
+// Canonical coefficients times e^{-i m phi_p}/(2 pi) (the reference's
+// normalisation, see the top): out[2*t], out[2*t+1] for
 // (i, j) = (0,0) (1,0) (0,1) (2,0) (1,1) (0,2).
 TEUK_HD void t4_coeffs(const StepIn &p, double *out) {
   const double M = p.mass, a = p.a, E = p.E, Lz = p.lz, em = p.mm;
@@ -348,8 +363,8 @@ TEUK_HD void t4_coeffs(const StepIn &p, double *out) {
       }
     }
   }
-  // times e^{-i m phi_p}/pi
-  const double ipi = 0.3183098861837907;
+  // times e^{-i m phi_p}/pi and the reference's factor 1/2 (exact: a power of 2)
+  const double ipi = 0.5 * 0.3183098861837907;
   const double mr = p.cmphi * ipi, mi = -p.smphi * ipi;
   for (int t = 0; t < dN; t++) {
     out[2 * t] = cr[t] * mr - ci[t] * mi;

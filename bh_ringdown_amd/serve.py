@@ -9,6 +9,9 @@ as HBM budget frees up.  After each chunk it streams the new psi4 / h_lm
 records of every resident solve back to the host and retires the solves
 that reached their final time.  A result does not depend on what else was
 in the batch: batched steps are bitwise the steps of a solve run alone.
+A solve that fails (the device sentinel found a non-finite or runaway value
+in its fields, teuk_status) is retired on its own with `failed` set; the
+other resident solves go on.
 
 The plain loop it is compared with runs the same requests one at a time,
 in arrival order, each as its own solver (one solve per launch).
@@ -48,6 +51,7 @@ class Result:
     points: int = 0
     records: dict = field(default_factory=dict)     # output(): k, t, dedt, h
     psi: np.ndarray = None
+    failed: bool = False    # retired by the divergence sentinel (records up to the failure)
 
     @property
     def latency(self):
@@ -131,7 +135,8 @@ class Server:
         now = lambda: clock() - t0
         # host time spent preparing requests, enqueuing chunks, waiting for
         # the GPU and streaming records back
-        st = self.stats = {"prepare_s": 0.0, "enqueue_s": 0.0, "sync_s": 0.0, "stream_s": 0.0, "chunks": 0}
+        st = self.stats = {"prepare_s": 0.0, "enqueue_s": 0.0, "sync_s": 0.0, "stream_s": 0.0, "chunks": 0,
+                           "failed": 0}
         batch = teuk.Batch(integrator=self.integrator, hbm_budget=self.hbm_budget)
         if self.tiling:
             batch.set_tiling(chunk=self.chunk, **self.tiling)
@@ -173,13 +178,28 @@ class Server:
                     if pending:
                         time.sleep(max(0.0, min(0.05, pending[0].arrival - now())))
                     continue
-                # 3. chunk done: stream records, retire finished solves
+                # 3. chunk done: stream records, retire finished and failed solves
                 ts = clock()
-                batch.sync()
+                try:
+                    batch.sync()
+                    any_failed = False
+                except teuk.TeukError as e:
+                    if e.code != teuk.TEUK_EDIVERGED:
+                        raise
+                    any_failed = True
                 st["sync_s"] += clock() - ts
                 ts = clock()
                 for rid in list(live):
                     req, p, s, nread = live[rid]
+                    if any_failed and s.status() == teuk.TEUK_EDIVERGED:
+                        res = results[rid]
+                        res.failed = True
+                        res.done = now()
+                        batch.remove(s)
+                        s.close()
+                        del live[rid]
+                        st["failed"] += 1
+                        continue
                     out = s.output(first=nread)
                     n_new = len(out["k"])
                     if n_new:
@@ -222,7 +242,12 @@ def run_plain(requests, clock=time.perf_counter, integrator=1):
         try:
             n = p.nsteps()
             s.step(n)
-            s.sync()
+            try:
+                s.sync()
+            except teuk.TeukError as e:
+                if e.code != teuk.TEUK_EDIVERGED:
+                    raise
+                res.failed = True
             res.records = s.output()
             res.psi = s.psi()
         finally:
@@ -235,10 +260,13 @@ def run_plain(requests, clock=time.perf_counter, integrator=1):
 
 
 def summary(results, wall=None):
-    """p50 / p99 latency, solves per hour and point-updates per second."""
-    done = [r for r in results.values() if not math.isnan(r.done)]
+    """p50 / p99 latency, solves per hour and point-updates per second of
+    the completed solves (failed ones are counted apart)."""
+    done = [r for r in results.values() if not math.isnan(r.done) and not r.failed]
+    failed = sum(1 for r in results.values() if r.failed)
     lat = np.array([r.latency for r in done])
     span = wall if wall is not None else max(r.done for r in done)
     pts = sum(r.points for r in done)
-    return {"solves": len(done), "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
-            "solves_per_hour": 3600.0 * len(done) / span, "point_updates_per_s": pts / span, "wall_s": span}
+    return {"solves": len(done), "failed": failed, "p50_s": float(np.percentile(lat, 50)),
+            "p99_s": float(np.percentile(lat, 99)), "solves_per_hour": 3600.0 * len(done) / span,
+            "point_updates_per_s": pts / span, "wall_s": span}

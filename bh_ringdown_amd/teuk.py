@@ -68,7 +68,7 @@ EXPORTS = [
     "teuk_set_tiling", "teuk_remaining", "teuk_batch_create", "teuk_batch_destroy", "teuk_batch_add",
     "teuk_batch_remove", "teuk_batch_set_tiling", "teuk_batch_step", "teuk_batch_sync", "teuk_batch_info",
     "teuk_source_coeffs", "teuk_source_patch", "teuk_grid_half", "teuk_create_staggered",
-    "teuk_trajectory_schedule_staggered",
+    "teuk_trajectory_schedule_staggered", "teuk_status",
 ]
 
 _lib = None
@@ -102,6 +102,7 @@ def lib():
         "teuk_set_trajectory": ([vp, _dp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "teuk_step": ([vp, ctypes.c_int], ctypes.c_int),
         "teuk_sync": ([vp], ctypes.c_int),
+        "teuk_status": ([vp], ctypes.c_int),
         "teuk_progress": ([vp, ctypes.POINTER(ctypes.c_int), _dp], ctypes.c_int),
         "teuk_read_output": ([vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OutputRow),
                               ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
@@ -284,6 +285,13 @@ class Solver:
 
     def sync(self):
         _check(lib().teuk_sync(self._h), "teuk_sync")
+
+    def status(self):
+        """TEUK_OK, or TEUK_EDIVERGED once the solve has failed (teuk_status)."""
+        rc = lib().teuk_status(self._h)
+        if rc not in (TEUK_OK, TEUK_EDIVERGED):
+            _check(rc, "teuk_status")
+        return rc
 
     def progress(self):
         k = ctypes.c_int(); t = ctypes.c_double()

@@ -25,7 +25,8 @@ extern "C" {
 #define TEUK_EINVAL     -1   /* bad argument or unsupported size            */
 #define TEUK_EHIP       -2   /* a HIP runtime call failed                    */
 #define TEUK_ENOMEM     -3   /* device or host allocation failed             */
-#define TEUK_EDIVERGED  -4   /* non-finite field detected by the sentinel    */
+#define TEUK_EDIVERGED  -4   /* the solve failed: a non-finite value, or one
+                                above 1e60 in magnitude, in its fields       */
 #define TEUK_EIO        -5   /* trajectory or waveform file error            */
 #define TEUK_ERANGE     -6   /* trajectory exhausted / step beyond schedule  */
 
@@ -136,7 +137,19 @@ int teuk_trajectory_schedule_staggered(const double *rows, int nrows, int nsteps
  * followed by its edge fill).  Asynchronous on the solver's stream; use
  * teuk_sync. */
 int teuk_step(teuk_solver *s, int nsteps);
+/* Waits for the solver's queued steps; TEUK_EDIVERGED once the solve has
+ * failed (see teuk_status). */
 int teuk_sync(teuk_solver *s);
+
+/* TEUK_OK, or TEUK_EDIVERGED once this solve has failed: after each chunk
+ * of steps (at most 32, or one teuk_step call) a device sentinel checks
+ * every value of the fields for non-finite values and magnitudes above
+ * 1e60, and each step checks Psi at the extraction column.  Works alone or
+ * in a batch (waits for the stream that steps the solver); a failed solve
+ * never touches another solve's memory, so its batch neighbours go on
+ * unchanged and the caller retires it with teuk_batch_remove.  No
+ * counterpart in the reference, which checks nothing (body.cu:105-216). */
+int teuk_status(teuk_solver *s);
 
 /* Number of completed steps and current solver time. */
 int teuk_progress(teuk_solver *s, int *k, double *t);
@@ -208,6 +221,8 @@ int teuk_batch_remove(teuk_batch *b, teuk_solver *s);
 int teuk_batch_set_tiling(teuk_batch *b, int waves_per_block, int waves_per_eu, int order, int chunk);
 /* Advance every unfinished member by up to nsteps steps (asynchronous). */
 int teuk_batch_step(teuk_batch *b, int nsteps);
+/* Waits for the batch; TEUK_EDIVERGED if any member has failed (find which
+ * with teuk_status on each). */
 int teuk_batch_sync(teuk_batch *b);
 int teuk_batch_info(teuk_batch *b, int *members, int *active, long long *bytes_used);
 

@@ -6,7 +6,9 @@ in source_derivation.py, evaluated at 30 significant digits.
 
 The C++ jet evaluator (teuk_t4.h, used by libteuk and by the oracle) is a
 separate implementation of the same equations; tests/test_source_math.py
-checks it against these vectors.
+checks it against these vectors.  The vectors carry the standard
+normalisation (Teukolsky's T_4 with delta(phi) -> e^{-i m phi_p}/pi); the
+evaluator returns the reference's, half of it.
 """
 import json
 import os

@@ -18,7 +18,8 @@ generated code:
   constants of the step; third time derivatives of r, theta are zero
   (body.cu:124 passes 0).
 * Azimuthal mode: delta(phi - phi_p(t)) -> e^{-i m phi_p}/pi and d_phi -> i m,
-  the normalisation of kern.cu:1270-1271.
+  the normalisation of kern.cu:1270-1271.  (The reference's generated T_4 as
+  a whole is half of the result: teuk_t4.h applies that factor.)
 
 The operators act on the delta functions through the particle's motion,
 d_t delta(r - r_p(t)) = -r_p' delta'(r - r_p), and the result is reduced to

@@ -69,3 +69,92 @@ def test_tuner_defaults_and_local_cache(tmp_path, monkeypatch):
     assert tune.best_solver_tiling(4096, 32, 1) == (defaults[other]["waves_per_block"],
                                                      defaults[other]["waves_per_eu"])
     assert tune.best_solver_tiling(1024, 16, 1) is None
+
+
+class _FakeSolver:
+    """Stands in for teuk.Solver in the server's host logic: steps advance a
+    counter, every step writes one record, and a planted failure step makes
+    the solve report TEUK_EDIVERGED from then on."""
+
+    def __init__(self, p, fail_at=None):
+        self.p, self.k, self.fail_at, self.closed = p, 0, fail_at, False
+
+    def advance(self, n):
+        self.k = min(self.p.nsteps(), self.k + n)
+
+    def status(self):
+        from bh_ringdown_amd import teuk
+        return teuk.TEUK_EDIVERGED if self.fail_at is not None and self.k >= self.fail_at else teuk.TEUK_OK
+
+    def output(self, first=0):
+        k = np.arange(first, self.k)
+        return {"k": k, "write": k * 0, "t": k * 1.0, "dedt": k * 0.0, "h": np.zeros((len(k), 14))}
+
+    def psi(self, first=0):
+        return np.zeros((max(0, self.k - first), 2, self.p.M))
+
+    def remaining(self):
+        return self.p.nsteps() - self.k
+
+    def close(self):
+        self.closed = True
+
+
+class _FakeBatch:
+    def __init__(self, integrator=1, hbm_budget=0):
+        self.members = []
+
+    def set_tiling(self, *a, **k):
+        pass
+
+    def add(self, s):
+        self.members.append(s)
+        return True
+
+    def remove(self, s):
+        self.members.remove(s)
+
+    def step(self, n):
+        for s in self.members:
+            s.advance(n)
+
+    def sync(self):
+        from bh_ringdown_amd import teuk
+        if any(s.status() == teuk.TEUK_EDIVERGED for s in self.members):
+            raise teuk.TeukError(teuk.TEUK_EDIVERGED, "teuk_batch_sync")
+
+    def info(self):
+        return {"members": len(self.members), "active": len(self.members), "bytes": 0}
+
+    def close(self):
+        pass
+
+
+def test_server_retires_a_failed_solve_and_keeps_the_others(monkeypatch):
+    """One resident solve diverges part-way: the server marks it failed and
+    closes it, and every other request still runs to its end with all of
+    its records (the batch's own failure report is per member)."""
+    from bh_ringdown_amd import teuk
+    reqs = [serve.Request(id=i, arrival=0.0, a=0.5, m=2, iota=0.1, eta=1e-3, N=4096, M=32,
+                          t_final=20.0 + 10.0 * i, seed=i) for i in range(4)]
+    made = {}
+
+    def fake_prepare(req, integrator=1):
+        p = serve.request_params(req, integrator)
+        s = _FakeSolver(p, fail_at=60 if req.id == 1 else None)
+        made[req.id] = s
+        return p, s
+
+    monkeypatch.setattr(serve, "prepare", fake_prepare)
+    monkeypatch.setattr(serve.teuk, "Batch", _FakeBatch)
+    srv = serve.Server(hbm_budget=1 << 40, chunk=16, target_points=1 << 30)
+    res = srv.run(reqs)
+    assert res[1].failed and not any(res[i].failed for i in (0, 2, 3))
+    assert srv.stats["failed"] == 1
+    assert made[1].closed and made[1].k < made[1].p.nsteps()
+    for i in (0, 2, 3):
+        p = made[i].p
+        assert made[i].closed and made[i].k == p.nsteps()
+        assert np.array_equal(res[i].records["k"], np.arange(p.nsteps()))
+    s = serve.summary(res, wall=1.0)
+    assert s["solves"] == 3 and s["failed"] == 1

@@ -1,12 +1,18 @@
 """Pins for the particle source (kern.cu:770-2683).
 
 The reference's source kernel cannot be built here (it needs the CUDA
-compiler), so its parity is unpinned; the source is pinned instead by:
+compiler).  The source is pinned by:
   * the symbolic derivation of the canonical T_4 coefficients
     (tests/golden/source_derivation.py, vectors in source_t4.json) against
-    the Taylor-jet evaluator teuk_t4.h that libteuk and the oracle run;
+    the Taylor-jet evaluator teuk_t4.h that libteuk and the oracle run; the
+    vectors carry the standard normalisation, the evaluator the reference's,
+    which is half of it (REF_NORM; read off kern.cu's delta''(r) coefficient
+    and measured on the reference's own plunge waveform:
+    tests/test_examples_pin.py, tests/test_gpu_examples.py);
   * the normalisation of the discretised delta functions on the grid;
-  * the portable exp used for the Gaussians.
+  * the portable exp used for the Gaussians;
+  * a circular orbit's waveform against the quadrupole formula
+    (tests/test_examples_pin.py).
 """
 import filecmp
 import json
@@ -20,13 +26,16 @@ from oracle import oracle as O
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLD = os.path.join(ROOT, "tests", "golden", "source_t4.json")
+# The reference's T_4 is half of Teukolsky's for the point-mass stress tensor
+# (kern.cu:2630 with :2552, :2363, :2281, :2280, :2262, :1688, :1368).
+REF_NORM = 0.5
 
 
 def test_jet_evaluator_matches_symbolic_derivation():
     data = json.load(open(GOLD))
     assert len(data["cases"]) >= 8
     for case in data["cases"]:
-        want = np.array(case["coeffs"])
+        want = REF_NORM * np.array(case["coeffs"])
         got = O.t4_coeffs(case["state"])
         # each (i, j) coefficient separately, relative to its own size
         for t in range(6):
@@ -41,7 +50,7 @@ def test_jet_evaluator_catches_a_wrong_sign():
     st = dict(case["state"])
     st["d2rdt2"] = -st["d2rdt2"] if st["d2rdt2"] else 0.01
     got = O.t4_coeffs(st)
-    want = np.array(case["coeffs"])
+    want = REF_NORM * np.array(case["coeffs"])
     assert np.linalg.norm(got - want) / np.linalg.norm(want) > 1e-6
 
 

@@ -89,7 +89,8 @@ int dalloc(teuk_solver *s, T **p, size_t count) {
 int validate(const teuk_params *p) {
   if (!p) return TEUK_EINVAL;
   if (p->N < 128 || p->N % kWave != 0) return TEUK_EINVAL;
-  if (p->M < 16 || p->M > 256 || p->M % kRows != 0) return TEUK_EINVAL;
+  if (p->M < 16 || p->M % kRows != 0) return TEUK_EINVAL;
+  if (p->M > (p->integrator == 2 ? 256 : 1024)) return TEUK_EINVAL;
   if ((long long)p->N * p->M > (1LL << 26)) return TEUK_EINVAL;  // kernels address planes with 32-bit offsets
   if (p->s != -2.0) return TEUK_EINVAL;  // kern.cu:142-145 bake in s = -2
   if (!(p->dt > 0.0) || !(p->dx > 0.0) || !(p->dtheta > 0.0)) return TEUK_EINVAL;

@@ -91,7 +91,8 @@ int teuk_default_params(teuk_params *p, int N, int M, double xmin, double xmax,
 /* ---- solver lifecycle ---------------------------------------------------- */
 /* Allocates device state, uploads r_c/theta, computes the Kerr coefficient
  * fields and zeroes the fields (kernel_init, kern.cu:84-169; body.cu:32-54).
- * N must be a multiple of 64 and >= 128; M a multiple of 8 and >= 16. */
+ * N must be a multiple of 64 and >= 128; M a multiple of 8, >= 16 and
+ * <= 1024 (<= 256 for the staggered scheme); N * M <= 2^26. */
 int teuk_create(const teuk_params *p, const double *r_c, const double *theta,
                 teuk_solver **out);
 

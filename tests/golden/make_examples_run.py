@@ -5,16 +5,19 @@ reference's waveform, in the reference's time frame (body.cu:207), for
 
   * "ref":  the grid of the shipped parm.h:1-8 and main.c:46 (N = 16000,
             M = 256, rho in [-50, 50], dt = 0.002, Fr = 50), SSP-RK3;
-  * "half": N = 8000, M = 128, dt = 0.004, Fr = 25 (the convergence check);
+  * "half": N = 8000, M = 128, dt = 0.004, Fr = 25;
+  * other grids named N:M:dt:Fr; the committed file also holds
+    16000:512:0.002:50 (the reference's production theta resolution, which
+    reproduces its waveform), 16000:128:0.002:50 and 8000:256:0.004:25;
 
-both on the trajectory examples_pin.solver_rows builds (source switched on
-at t = 2140).  About 100 s of GPU time:
+all on the trajectory examples_pin.solver_rows builds (source switched on at
+t = 2140).  GPU time: ~85 s for "ref", ~170 s for 16000:512, ~10 s for
+"half":
 
-    python tests/golden/make_examples_run.py [ref half ...]
+    python tests/golden/make_examples_run.py ref half 16000:512:0.002:50 ...
 
 tests/test_examples_pin.py checks the records against the reference's own
-waveform and QNM amplitudes.  Extra grids (for the convergence study in the
-design notes) can be named as N:M:dt:Fr.
+waveform and QNM amplitudes; tests/test_gpu_examples.py reruns "half".
 """
 import os
 import sys

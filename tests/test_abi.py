@@ -51,6 +51,7 @@ def test_create_rejects_bad_arguments_before_touching_the_gpu():
     bad = [
         dict(N=100), dict(M=12), dict(M=20), dict(s=-1.0), dict(Fr=0), dict(integrator=3),
         dict(j_extract=5000), dict(a=1.0), dict(m=1.5), dict(dt=0.0),
+        dict(M=1032), dict(M=264, integrator=2), dict(N=65536, M=1032),
     ]
     for kw in bad:
         p = teuk.default_params(1024, 32, -100.0, 300.0, 0.5, 2, T=10.0)

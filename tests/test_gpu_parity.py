@@ -124,9 +124,15 @@ def test_plain_1024x32_full_run(integrator):
     # theta chunk counts 3, 5, 6 (default launch shapes of 1, 1, 2 waves per
     # block) and the largest M with the smallest N
     (128, 24, -25.0, 25.0, 0.5), (256, 40, -50.0, 50.0, 0.5), (256, 48, -50.0, 50.0, 0.5),
-    (128, 256, -25.0, 25.0, 0.25)])
+    (128, 256, -25.0, 25.0, 0.25),
+    # finer theta grids than the reference's M = 256 (dt / dtheta kept below 3.6)
+    (128, 512, -25.0, 25.0, 0.05), (256, 1024, -50.0, 50.0, 0.02)])
 def test_sizes(N, M, xmin, xmax, cfl):
     check(make(N, M, xmin, xmax, 0.5, 2, 60, dt_over_dx=cfl), 60)
+
+
+def test_rk4_fine_theta_grid():
+    check(make(256, 512, -50.0, 50.0, 0.7, -2, 48, integrator=1, Fr=3, dt_over_dx=0.04), 48)
 
 
 @pytest.mark.parametrize("m", [-3, -2, 0, 1, 3, 4])

@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <utility>
@@ -313,6 +314,7 @@ const char *teuk_strerror(int code) {
     case TEUK_EDIVERGED: return "non-finite field (solve diverged)";
     case TEUK_EIO: return "trajectory or waveform file error";
     case TEUK_ERANGE: return "step beyond the scheduled run or trajectory";
+    case TEUK_ETOOBIG: return "the solve needs more than teuk_solve_noalloc's fixed scratch memory";
     default: return "unknown error";
   }
 }
@@ -395,16 +397,141 @@ int teuk_create_staggered(const teuk_params *p, const double *r_h, const double 
 
 namespace {
 
-// Device state, tables and coefficient set-up of one solver; r_h only for
-// the staggered scheme.
-int create_solver(const teuk_params *p, const double *r_h, const double *r_c, const double *theta,
-                  teuk_solver **out) {
-  int rc = TEUK_OK;
-  teuk_solver *s = new teuk_solver();
-  s->P = *p;
-  s->nt = scheduled_steps(p);
-  s->tiling = default_tiling(p->M);
-  Grid &g = s->g;
+// Byte offsets of one solver's pieces in its device memory, each piece
+// 256-byte aligned: the tables copied from the host first (one copy), then
+// the memory that starts at zero (one fill), then the rest.
+struct Layout {
+  size_t phase = 0, theta = 0, r_c = 0, ktab = 0, rtab = 0, extw = 0, projw = 0, org = 0, ctl = 0;
+  size_t phase_h = 0, r_h = 0, ktab_h = 0;   // staggered scheme only
+  size_t init_end = 0;
+  size_t states = 0, integ = 0, patch = 0;
+  size_t zero_end = 0;
+  size_t dbdx = 0, scratch = 0, snap = 0, rec = 0, rec_k = 0, dbdx_h = 0;
+  size_t end = 0;
+};
+
+size_t piece(size_t &at, size_t bytes) {
+  const size_t off = at;
+  at += (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
+  return off;
+}
+
+int state_count(int integrator) { return integrator == 0 ? 3 : (integrator == 1 ? 4 : 2); }
+
+Layout plan_layout(const teuk_params *p, int nrec_cap) {
+  Layout L;
+  const bool stag = p->integrator == 2;
+  const size_t N = (size_t)p->N, M = (size_t)p->M, NM = N * M, D = sizeof(double);
+  size_t at = 0;
+  L.phase = piece(at, 2 * N * D);
+  L.theta = piece(at, M * D);
+  L.r_c = piece(at, N * D);
+  L.ktab = piece(at, kNK * N * D);
+  L.rtab = piece(at, kNR * M * D);
+  L.extw = piece(at, (2 * M + 1) * D);
+  L.projw = piece(at, 7 * (5 * (M / 4)) * 3 * D);
+  L.org = piece(at, 2 * kRing * sizeof(int));
+  L.ctl = piece(at, sizeof(Ctl));
+  if (stag) {
+    L.phase_h = piece(at, 2 * N * D);
+    L.r_h = piece(at, N * D);
+    L.ktab_h = piece(at, kNK * N * D);
+  }
+  L.init_end = at;
+  // Zero fields and integrals (kern.cu:164-167, body.cu:86-91), empty patches.
+  L.states = piece(at, (size_t)state_count(p->integrator) * 4 * NM * D);
+  L.integ = piece(at, 4 * M * D);
+  L.patch = piece(at, (size_t)kRing * 2 * p->ptsx * p->ptsy * D);
+  L.zero_end = at;
+  L.dbdx = piece(at, NM * D);
+  L.scratch = piece(at, NM * D);
+  L.snap = piece(at, (size_t)nrec_cap * 6 * M * D);
+  L.rec = piece(at, (size_t)nrec_cap * 16 * D);
+  L.rec_k = piece(at, (size_t)nrec_cap * 2 * sizeof(int));
+  if (stag) L.dbdx_h = piece(at, NM * D);
+  L.end = at;
+  return L;
+}
+
+// Host contents of the copied pieces, written at their offsets into `stage`
+// (L.init_end bytes): source phases, coefficient column and row tables,
+// extraction weights (body.cu:175-183, rr the extraction radius), projection
+// weights, patch origins, the control block; for the staggered scheme the
+// same at the half points (rtab_h: kNR * M doubles of workspace).
+void fill_init(const teuk_params *p, const double *r_c, const double *theta, const double *r_h,
+               double *rtab_h, const Layout &L, char *stage) {
+  auto at = [&](size_t off) { return reinterpret_cast<double *>(stage + off); };
+  const int N = p->N, M = p->M;
+  const bool stag = p->integrator == 2;
+  host_source_phase(N, p->mass, p->a, p->m, r_c, at(L.phase));
+  std::memcpy(at(L.theta), theta, (size_t)M * sizeof(double));
+  std::memcpy(at(L.r_c), r_c, (size_t)N * sizeof(double));
+  host_coeff_tables(N, M, p->xmin, p->xmax, p->dx, p->mass, p->a, p->m, r_c, theta, at(L.ktab), at(L.rtab), 0.0,
+                    stag);
+  double *w = at(L.extw);
+  const double rr = r_c[p->j_extract < 0 ? N - 1 : p->j_extract];
+  for (int q = 0; q < M; q++) {
+    const double rho2 = rr * rr + std::pow(p->a * std::cos(theta[q]), 2);
+    w[q] = 0.5 * p->dtheta * std::sin(theta[q]) * rr * rr;
+    w[M + q] = std::pow(rho2, 4);
+  }
+  w[2 * M] = std::pow(rr, 6);
+  host_projection_weights(M, p->m, at(L.projw));
+  int *org = reinterpret_cast<int *>(stage + L.org);
+  for (int i = 0; i < 2 * kRing; i++) org[i] = -(1 << 29);
+  Ctl c0{};
+  c0.k = 0;
+  c0.nrec = 0;
+  c0.status = 0;
+  c0.timer = 0.0;
+  std::memcpy(stage + L.ctl, &c0, sizeof(Ctl));
+  if (stag) {  // Old/kern.cu:90-135: the same fields at rho - dx/2 with r_h
+    host_source_phase(N, p->mass, p->a, p->m, r_h, at(L.phase_h));
+    std::memcpy(at(L.r_h), r_h, (size_t)N * sizeof(double));
+    host_coeff_tables(N, M, p->xmin, p->xmax, p->dx, p->mass, p->a, p->m, r_h, theta, at(L.ktab_h), rtab_h, -0.5,
+                      true);
+  }
+}
+
+// Point a solver's device pointers into memory laid out by L at `base`.
+void bind_layout(const teuk_params *p, const Layout &L, char *base, DevState &d) {
+  auto at = [&](size_t off) { return reinterpret_cast<double *>(base + off); };
+  const size_t NM = (size_t)p->N * p->M;
+  const int nstates = state_count(p->integrator);
+  d.phase = at(L.phase);
+  d.theta = at(L.theta);
+  d.r_c = at(L.r_c);
+  d.ktab = at(L.ktab);
+  d.rtab = at(L.rtab);
+  d.ext.extw = at(L.extw);
+  d.ext.projw = at(L.projw);
+  d.patch_org = reinterpret_cast<int *>(base + L.org);
+  d.ctl = reinterpret_cast<Ctl *>(base + L.ctl);
+  double *states = at(L.states);
+  d.U = states;
+  d.C = states + 4 * NM;   // staggered scheme: the half-point fields
+  d.D = nstates >= 3 ? states + 8 * NM : nullptr;
+  d.A = nstates == 4 ? states + 12 * NM : nullptr;
+  d.ext.integ = at(L.integ);
+  d.patch = at(L.patch);
+  d.dbdx = at(L.dbdx);
+  d.scratch = at(L.scratch);
+  d.ext.snap = at(L.snap);
+  d.ext.rec = at(L.rec);
+  d.ext.rec_k = reinterpret_cast<int *>(base + L.rec_k);
+  if (p->integrator == 2) {
+    d.phase_h = at(L.phase_h);
+    d.r_h = at(L.r_h);
+    d.ktab_h = at(L.ktab_h);
+    d.dbdx_h = at(L.dbdx_h);
+  }
+  d.sched = nullptr;
+  d.nsched = 0;
+}
+
+// The run description of a solve.
+Grid make_grid(const teuk_params *p, int nrec_cap) {
+  Grid g{};
   g.N = p->N;
   g.M = p->M;
   g.even_m = (((int)p->m) % 2) == 0;
@@ -412,7 +539,7 @@ int create_solver(const teuk_params *p, const double *r_h, const double *r_c, co
   g.ptsy = p->ptsy;
   g.je = p->j_extract < 0 ? p->N - 1 : p->j_extract;
   g.Fr = p->Fr;
-  g.nrec_cap = s->nt / p->Fr + 1;
+  g.nrec_cap = nrec_cap;
   g.xmin = p->xmin;
   g.xmax = p->xmax;
   g.dx = p->dx;
@@ -423,121 +550,43 @@ int create_solver(const teuk_params *p, const double *r_h, const double *r_c, co
   g.m = p->m;
   g.start = 0.0;
   g.stag = p->integrator == 2 ? 1 : 0;
+  return g;
+}
 
+// Device state, tables and coefficient set-up of one solver; r_h only for
+// the staggered scheme.  All of the solver's device memory is one
+// allocation (plan_layout); its tables go up in one copy from one staging
+// buffer and the memory that starts at zero is cleared by one fill.
+int create_solver(const teuk_params *p, const double *r_h, const double *r_c, const double *theta,
+                  teuk_solver **out) {
+  int rc = TEUK_OK;
+  teuk_solver *s = new teuk_solver();
+  s->P = *p;
+  s->nt = scheduled_steps(p);
+  s->tiling = default_tiling(p->M);
+  s->g = make_grid(p, s->nt / p->Fr + 1);
+  Grid &g = s->g;
   auto fail = [&](int code) {
     teuk_destroy(s);
     return code;
   };
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return fail(TEUK_EHIP);
-  const size_t NM = (size_t)p->N * p->M;
-  DevState &d = s->d;
-  const int nstates = p->integrator == 0 ? 3 : (p->integrator == 1 ? 4 : 2);
-  const size_t nproj = 7 * (size_t)(5 * (p->M / 4)) * 3;
-
-  // Host-side contents of the tables: source phases, extraction weights
-  // (body.cu:175-183, rr is the extraction radius), coefficient column and
-  // row tables, projection weights, patch origins, the control block.
-  std::vector<double> ph(2 * (size_t)p->N), ph_h;
-  host_source_phase(p->N, p->mass, p->a, p->m, r_c, ph.data());
-  if (g.stag) {
-    ph_h.resize(ph.size());
-    host_source_phase(p->N, p->mass, p->a, p->m, r_h, ph_h.data());
-  }
-  std::vector<double> h_extw(2 * p->M + 1);
-  const double rr = r_c[g.je];
-  for (int q = 0; q < p->M; q++) {
-    const double rho2 = rr * rr + std::pow(p->a * std::cos(theta[q]), 2);
-    h_extw[q] = 0.5 * p->dtheta * std::sin(theta[q]) * rr * rr;
-    h_extw[p->M + q] = std::pow(rho2, 4);
-  }
-  h_extw[2 * p->M] = std::pow(rr, 6);
-  std::vector<double> ktab(kNK * (size_t)p->N), rtab(kNR * (size_t)p->M);
-  host_coeff_tables(p->N, p->M, p->xmin, p->xmax, p->dx, p->mass, p->a, p->m, r_c, theta, ktab.data(),
-                    rtab.data(), 0.0, g.stag != 0);
-  std::vector<double> ktab_h, rtab_h;
-  if (g.stag) {  // Old/kern.cu:90-135: the same fields at rho - dx/2 with r_h
-    ktab_h.resize(ktab.size());
-    rtab_h.resize(rtab.size());
-    host_coeff_tables(p->N, p->M, p->xmin, p->xmax, p->dx, p->mass, p->a, p->m, r_h, theta, ktab_h.data(),
-                      rtab_h.data(), -0.5, true);
-  }
-  std::vector<double> h_projw(nproj);
-  host_projection_weights(p->M, p->m, h_projw.data());
-  std::vector<int> org(2 * kRing, -(1 << 29));
-  Ctl c0{};
-  c0.k = 0;
-  c0.nrec = 0;
-  c0.status = 0;
-  c0.timer = g.start;
-
-  // All of the solver's device memory is one allocation, carved in 256-byte
-  // aligned pieces: the tables copied from the host first (one copy), then
-  // the memory that starts at zero (one fill), then the rest.
-  size_t bytes = 0;
-  std::vector<std::function<void(char *)>> bind;   // point each piece into the allocation
-  auto carve = [&](auto **dst, size_t count) {
-    using T = std::remove_reference_t<decltype(**dst)>;
-    const size_t off = bytes;
-    bind.push_back([dst, off](char *b) { *dst = reinterpret_cast<T *>(b + off); });
-    bytes += (std::max<size_t>(count, 1) * sizeof(T) + 255) & ~(size_t)255;
-    return off;
-  };
-  std::vector<std::pair<size_t, std::pair<const void *, size_t>>> init;   // offset, host source, bytes
-  auto carve_init = [&](auto **dst, const auto *src, size_t count) {
-    using T = std::remove_reference_t<decltype(**dst)>;
-    init.push_back({carve(dst, count), {static_cast<const void *>(src), count * sizeof(T)}});
-  };
-  double *extw = nullptr, *projw = nullptr;
-  carve_init(&d.phase, ph.data(), ph.size());
-  carve_init(&d.theta, theta, (size_t)p->M);
-  carve_init(&d.r_c, r_c, (size_t)p->N);
-  carve_init(&d.ktab, ktab.data(), ktab.size());
-  carve_init(&d.rtab, rtab.data(), rtab.size());
-  carve_init(&extw, h_extw.data(), h_extw.size());
-  carve_init(&projw, h_projw.data(), h_projw.size());
-  carve_init(&d.patch_org, org.data(), org.size());
-  carve_init(&d.ctl, &c0, 1);
-  if (g.stag) {
-    carve_init(&d.phase_h, ph_h.data(), ph_h.size());
-    carve_init(&d.r_h, r_h, (size_t)p->N);
-    carve_init(&d.ktab_h, ktab_h.data(), ktab_h.size());
-  }
-  const size_t init_bytes = bytes;
-  double *states = nullptr;
-  // Zero fields and integrals (kern.cu:164-167, body.cu:86-91), empty patches.
-  carve(&states, nstates * 4 * NM);
-  carve(&d.ext.integ, 4 * (size_t)p->M);
-  carve(&d.patch, (size_t)kRing * 2 * p->ptsx * p->ptsy);
-  const size_t zero_end = bytes;
-  carve(&d.dbdx, NM);
-  carve(&d.scratch, NM);
-  carve(&d.ext.snap, (size_t)g.nrec_cap * 6 * p->M);
-  carve(&d.ext.rec, (size_t)g.nrec_cap * 16);
-  carve(&d.ext.rec_k, (size_t)g.nrec_cap * 2);
-  if (g.stag) carve(&d.dbdx_h, NM);
-
+  const Layout L = plan_layout(p, g.nrec_cap);
   char *base = nullptr;
-  if ((rc = dalloc(s, &base, bytes))) return fail(rc);
-  for (const auto &f : bind) f(base);
-  d.U = states;
-  d.C = states + 4 * NM;   // staggered scheme: the half-point fields
-  d.D = nstates >= 3 ? states + 8 * NM : nullptr;
-  d.A = nstates == 4 ? states + 12 * NM : nullptr;
-  d.ext.extw = extw;
-  d.ext.projw = projw;
-  d.sched = nullptr;
-  d.nsched = 0;
+  if ((rc = dalloc(s, &base, L.end))) return fail(rc);
+  bind_layout(p, L, base, s->d);
   {
-    std::vector<char> staging(init_bytes, 0);
-    for (const auto &e : init) std::memcpy(staging.data() + e.first, e.second.first, e.second.second);
-    if (hipMemcpyAsync(base, staging.data(), init_bytes, hipMemcpyHostToDevice, s->stream) != hipSuccess)
+    std::vector<char> staging(L.init_end, 0);
+    std::vector<double> rtab_h(g.stag ? kNR * (size_t)p->M : 0);
+    fill_init(p, r_c, theta, r_h, rtab_h.data(), L, staging.data());
+    if (hipMemcpyAsync(base, staging.data(), L.init_end, hipMemcpyHostToDevice, s->stream) != hipSuccess)
       return fail(TEUK_EHIP);
-    if (hipMemsetAsync(base + init_bytes, 0, zero_end - init_bytes, s->stream) != hipSuccess)
+    if (hipMemsetAsync(base + L.init_end, 0, L.zero_end - L.init_end, s->stream) != hipSuccess)
       return fail(TEUK_EHIP);
     // the staging buffer is pageable: the copy has completed when the call returns
     if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(TEUK_EHIP);
   }
-  if ((g.stag ? launch_init_so2(g, d, s->stream) : launch_init_coeffs(g, d, s->stream)) != hipSuccess)
+  if ((g.stag ? launch_init_so2(g, s->d, s->stream) : launch_init_coeffs(g, s->d, s->stream)) != hipSuccess)
     return fail(TEUK_EHIP);
   if (hipStreamSynchronize(s->stream) != hipSuccess) return fail(TEUK_EHIP);
   *out = s;
@@ -1170,4 +1219,155 @@ int teuk_source_patch(teuk_solver *s, int k, double *patch, int *origin) {
                    hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(origin, s->d.patch_org + 2 * (k % kRing), 2 * sizeof(int), hipMemcpyDeviceToHost));
   return TEUK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// A whole solve in fixed, preallocated memory (teuk_solve_noalloc).  All of
+// its device memory is `scratch` below, a file-scope device array that each
+// GPU holds from the moment the library is loaded on it until the process
+// ends; host-side staging uses a file-scope host array of the same size.
+// Calls are serialised by one process-wide lock (they share both arrays), so
+// two calls never overlap; the arrays keep the last call's contents, and
+// every call clears what must start at zero.
+namespace {
+
+constexpr size_t kNoAllocBytes = (size_t)TEUK_NOALLOC_BYTES;
+constexpr int kNoAllocDevices = 64;
+
+static __device__ __align__(256) char scratch[kNoAllocBytes];
+alignas(256) char g_noalloc_stage[kNoAllocBytes];
+void *g_noalloc_dev[kNoAllocDevices];   // `scratch` on each device, from hipGetSymbolAddress
+std::mutex g_noalloc_lock;
+
+// The device address of `scratch` on the current device: asked of the
+// runtime once per device (with that device current), then kept.  Host code
+// uses only this address for the array's memory.
+int noalloc_base(char **out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocDevices) return TEUK_EHIP;
+  if (!g_noalloc_dev[dev]) {
+    void *q = nullptr;
+    if (hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)) != hipSuccess || !q) return TEUK_EHIP;
+    g_noalloc_dev[dev] = q;
+  }
+  *out = static_cast<char *>(g_noalloc_dev[dev]);
+  return TEUK_OK;
+}
+
+size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace
+
+int teuk_solve_noalloc(const teuk_params *p, const double *r_c, const double *theta, const double *u0,
+                       const teuk_source_step *steps, int nsteps, double *u_out, teuk_output_row *rows,
+                       int max_rows, int *nrows) {
+  if (nrows) *nrows = 0;
+  if (!p || !r_c || !theta || nsteps < 0 || max_rows < 0 || (max_rows > 0 && !rows)) return TEUK_EINVAL;
+  int rc = validate(p);
+  if (rc) return rc;
+  if (p->integrator == 2) return TEUK_EINVAL;   // the RK integrators only
+  if (nsteps > scheduled_steps(p)) return TEUK_ERANGE;
+  if (steps && nsteps > 0) {
+    if (!source_available()) return TEUK_EINVAL;
+    const double rpl = p->mass + std::sqrt(p->mass * p->mass - p->a * p->a);
+    for (int k = 0; k < nsteps; k++)
+      if (!(steps[k].rp > rpl)) return TEUK_EINVAL;
+  }
+  const bool has_src = steps && nsteps > 0;
+  const size_t NM = (size_t)p->N * p->M;
+  const int nrec_cap = nsteps / p->Fr + 1;
+  const Layout L = plan_layout(p, nrec_cap);
+  // the source schedule after the solver's pieces: SrcDev entries, then the
+  // particle states the device turns into T_4 coefficients
+  const size_t off_sched = L.end;
+  const size_t b_dev = has_src ? round256((size_t)nsteps * sizeof(SrcDev)) : 0;
+  const size_t b_in = has_src ? round256((size_t)nsteps * sizeof(teuk_t4::StepIn)) : 0;
+  if (off_sched + b_dev + b_in > kNoAllocBytes) return TEUK_ETOOBIG;
+
+  std::lock_guard<std::mutex> hold(g_noalloc_lock);
+  char *base = nullptr;
+  if ((rc = noalloc_base(&base))) return rc;
+  char *stage = g_noalloc_stage;
+  const hipStream_t st = nullptr;
+  const Grid g = make_grid(p, nrec_cap);
+  DevState d{};
+  bind_layout(p, L, base, d);
+
+  // tables up, the zero region cleared, coefficient set-up (kernel_init)
+  std::memset(stage, 0, L.init_end);
+  fill_init(p, r_c, theta, nullptr, nullptr, L, stage);
+  HIPCHK(hipMemcpy(base, stage, L.init_end, hipMemcpyHostToDevice));
+  HIPCHK(hipMemsetAsync(base + L.init_end, 0, L.zero_end - L.init_end, st));
+  HIPCHK(launch_init_coeffs(g, d, st));
+  // initial fields as (re, im) pairs of Psi and Pi (teuk_set_fields); zero if none
+  if (u0) {
+    double *dev = reinterpret_cast<double *>(stage);
+    for (size_t h = 0; h < 2; h++)
+      for (size_t c = 0; c < NM; c++) {
+        dev[h * 2 * NM + 2 * c] = u0[(2 * h) * NM + c];
+        dev[h * 2 * NM + 2 * c + 1] = u0[(2 * h + 1) * NM + c];
+      }
+    HIPCHK(hipMemcpy(d.U, stage, 4 * NM * sizeof(double), hipMemcpyHostToDevice));
+  }
+  // source schedule (teuk_set_source)
+  if (has_src) {
+    SrcDev *hdev = reinterpret_cast<SrcDev *>(stage);
+    teuk_t4::StepIn *hin = reinterpret_cast<teuk_t4::StepIn *>(stage + b_dev);
+    const SrcStep *raw = reinterpret_cast<const SrcStep *>(steps);
+    for (int k = 0; k < nsteps; k++) {
+      SrcPrep prep;
+      host_source_prepare(g, raw[k], prep);
+      hdev[k] = prep.dev;
+      hin[k] = prep.in;
+    }
+    HIPCHK(hipMemcpy(base + off_sched, stage, b_dev + b_in, hipMemcpyHostToDevice));
+    d.sched = reinterpret_cast<SrcDev *>(base + off_sched);
+    d.nsched = nsteps;
+    HIPCHK(launch_source_coeffs(reinterpret_cast<const teuk_t4::StepIn *>(base + off_sched + b_dev), d.sched,
+                                nsteps, st));
+  }
+  // the steps, in the launch sequence of teuk_step: chunks of kRing steps
+  // (patches for the chunk, the fused stages of each step), each closed by
+  // the sentinel and the output records
+  const Tiling tiling = default_tiling(p->M);
+  const int nstage = stages_per_step(p->integrator);
+  for (int done = 0; done < nsteps;) {
+    const int n = std::min(kRing, nsteps - done);
+    for (int i = 0; i < n; i++) {
+      if (has_src && i == 0) HIPCHK(launch_source(g, d, 0, kRing, st));
+      for (int s = 1; s <= nstage; s++) HIPCHK(launch_stage(g, d, p->integrator, s, has_src, i, tiling, st));
+    }
+    HIPCHK(launch_sentinel(g, d, st));
+    HIPCHK(launch_finalize(g, d, n, st));
+    done += n;
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  // results back (teuk_get_fields, teuk_read_output)
+  Ctl c;
+  HIPCHK(hipMemcpy(&c, d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+  if (u_out) {
+    const double *dev = reinterpret_cast<const double *>(stage);
+    HIPCHK(hipMemcpy(stage, d.U, 4 * NM * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t h = 0; h < 2; h++)
+      for (size_t q = 0; q < NM; q++) {
+        u_out[(2 * h) * NM + q] = dev[h * 2 * NM + 2 * q];
+        u_out[(2 * h + 1) * NM + q] = dev[h * 2 * NM + 2 * q + 1];
+      }
+  }
+  const int cnt = std::max(0, std::min(max_rows, c.nrec));
+  if (cnt > 0) {
+    const double *rec = reinterpret_cast<const double *>(stage);
+    const int *rk = reinterpret_cast<const int *>(stage + round256((size_t)cnt * 16 * sizeof(double)));
+    HIPCHK(hipMemcpy(stage, d.ext.rec, (size_t)cnt * 16 * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(const_cast<int *>(rk), d.ext.rec_k, (size_t)cnt * 2 * sizeof(int), hipMemcpyDeviceToHost));
+    for (int i = 0; i < cnt; i++) {
+      rows[i].k = rk[2 * i];
+      rows[i].write = rk[2 * i + 1];
+      rows[i].t = rec[16 * i];
+      rows[i].dedt = rec[16 * i + 1];
+      for (int j = 0; j < 14; j++) rows[i].h[j] = rec[16 * i + 2 + j];
+    }
+  }
+  if (nrows) *nrows = cnt;
+  return c.status ? TEUK_EDIVERGED : TEUK_OK;
 }

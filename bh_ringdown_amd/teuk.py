@@ -20,6 +20,8 @@ PI15 = 3.14159265358979  # parm.h:12
 _dp = ctypes.POINTER(ctypes.c_double)
 
 TEUK_OK, TEUK_EINVAL, TEUK_EHIP, TEUK_ENOMEM, TEUK_EDIVERGED, TEUK_EIO, TEUK_ERANGE = 0, -1, -2, -3, -4, -5, -6
+TEUK_ETOOBIG = -7
+NOALLOC_BYTES = 64 << 20
 
 
 class TeukError(RuntimeError):
@@ -68,7 +70,7 @@ EXPORTS = [
     "teuk_set_tiling", "teuk_remaining", "teuk_batch_create", "teuk_batch_destroy", "teuk_batch_add",
     "teuk_batch_remove", "teuk_batch_set_tiling", "teuk_batch_step", "teuk_batch_sync", "teuk_batch_info",
     "teuk_source_coeffs", "teuk_source_patch", "teuk_grid_half", "teuk_create_staggered",
-    "teuk_trajectory_schedule_staggered", "teuk_status",
+    "teuk_trajectory_schedule_staggered", "teuk_status", "teuk_solve_noalloc",
 ]
 
 _lib = None
@@ -103,6 +105,9 @@ def lib():
         "teuk_step": ([vp, ctypes.c_int], ctypes.c_int),
         "teuk_sync": ([vp], ctypes.c_int),
         "teuk_status": ([vp], ctypes.c_int),
+        "teuk_solve_noalloc": ([P, _dp, _dp, _dp, ctypes.POINTER(SourceStep), ctypes.c_int, _dp,
+                                ctypes.POINTER(OutputRow), ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+                               ctypes.c_int),
         "teuk_progress": ([vp, ctypes.POINTER(ctypes.c_int), _dp], ctypes.c_int),
         "teuk_read_output": ([vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OutputRow),
                               ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
@@ -405,6 +410,39 @@ class Batch:
         m = ctypes.c_int(); a = ctypes.c_int(); b = ctypes.c_longlong()
         _check(lib().teuk_batch_info(self._h, ctypes.byref(m), ctypes.byref(a), ctypes.byref(b)), "teuk_batch_info")
         return {"members": m.value, "active": a.value, "bytes": b.value}
+
+
+def solve_noalloc(p, r_c, theta, nsteps, u0=None, steps=None, max_rows=None):
+    """teuk_solve_noalloc: a whole solve in the library's fixed scratch memory.
+    Returns (code, fields[4, M, N], records as Solver.output()); code is
+    TEUK_OK, TEUK_EDIVERGED or TEUK_ETOOBIG (then fields and records are
+    empty); other errors raise."""
+    r_c = np.ascontiguousarray(r_c, dtype=np.float64)
+    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    if u0 is not None and np.shape(u0) != (4, p.M, p.N):
+        raise ValueError("fields must have shape (4, M, N)")
+    u0c = np.ascontiguousarray(u0, dtype=np.float64) if u0 is not None else None   # kept alive for the call
+    u0p = _ptr(u0c) if u0c is not None else None
+    arr = None
+    if steps is not None and len(steps):
+        if len(steps) != nsteps:
+            raise ValueError("one source entry per step")
+        arr = (SourceStep * len(steps))(*steps)
+    cap = nsteps // p.Fr + 1 if max_rows is None else max_rows
+    rows = (OutputRow * max(cap, 1))()
+    u = np.zeros((4, p.M, p.N))
+    n = ctypes.c_int()
+    rc = lib().teuk_solve_noalloc(ctypes.byref(p), _ptr(r_c), _ptr(theta), u0p, arr, int(nsteps), _ptr(u), rows,
+                                  cap, ctypes.byref(n))
+    if rc not in (TEUK_OK, TEUK_EDIVERGED, TEUK_ETOOBIG):
+        _check(rc, "teuk_solve_noalloc")
+    k = n.value
+    out = {"k": np.array([rows[i].k for i in range(k)], dtype=np.int64),
+           "write": np.array([rows[i].write for i in range(k)], dtype=np.int64),
+           "t": np.array([rows[i].t for i in range(k)]),
+           "dedt": np.array([rows[i].dedt for i in range(k)]),
+           "h": np.array([list(rows[i].h) for i in range(k)]).reshape(k, 14)}
+    return rc, u, out
 
 
 def body(p, r_c, theta, traj_path=None, out_path=None, full_path=None, verbose=False):

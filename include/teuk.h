@@ -29,6 +29,8 @@ extern "C" {
                                 above 1e60 in magnitude, in its fields       */
 #define TEUK_EIO        -5   /* trajectory or waveform file error            */
 #define TEUK_ERANGE     -6   /* trajectory exhausted / step beyond schedule  */
+#define TEUK_ETOOBIG    -7   /* more memory than teuk_solve_noalloc's fixed
+                                scratch (TEUK_NOALLOC_BYTES)                 */
 
 /* ---- run parameters ---------------------------------------------------- */
 /* Field names follow parm.h:1-11 and main.c:55-61 of the reference. */
@@ -252,6 +254,31 @@ int teuk_body(const teuk_params *p, const double *r_c, const double *theta,
  * part of the reference tree. */
 int teuk_traj_plunge(double a, double eta, double iota, double r0_over_isco, double dt_out,
                      double t_final, unsigned long long seed, double *rows, int cap, int *nrows);
+
+/* ---- a solve that allocates nothing --------------------------------------- */
+/* No counterpart in the reference.  For callers that cannot allocate memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation): one whole solve -- what teuk_create, teuk_set_fields,
+ * teuk_set_source, teuk_step(nsteps), teuk_sync, teuk_get_fields,
+ * teuk_read_output and teuk_destroy do together, with the same kernels and
+ * bit for bit the same results -- that calls no allocating routine, host or
+ * device.  All of its device memory (fields, tables, schedule, records) is a
+ * fixed scratch array of TEUK_NOALLOC_BYTES that each GPU the library runs
+ * on holds until the process ends; a solve needing more returns
+ * TEUK_ETOOBIG (e.g. RK4 at 4096 x 32 fits with a 6000-step schedule and
+ * Fr >= 2; 16384 x 64 does not).  Integrators 0 and 1.
+ *   u0:    initial fields (layout of teuk_set_fields), NULL = zero fields
+ *   steps: nsteps source entries (teuk_set_source), NULL = no source
+ *   u_out: final fields (teuk_get_fields layout), may be NULL
+ *   rows:  up to max_rows output records (teuk_read_output); *nrows = count
+ * Calls are serialised by a process-wide lock (they share the scratch, which
+ * keeps the last call's contents; each call clears what must start at
+ * zero).  Runs on the current device's null stream and returns when done;
+ * TEUK_EDIVERGED if the sentinel fired. */
+#define TEUK_NOALLOC_BYTES (64LL << 20)
+int teuk_solve_noalloc(const teuk_params *p, const double *r_c, const double *theta,
+                       const double *u0, const teuk_source_step *steps, int nsteps,
+                       double *u_out, teuk_output_row *rows, int max_rows, int *nrows);
 
 /* Human-readable text for an error code. */
 const char *teuk_strerror(int code);

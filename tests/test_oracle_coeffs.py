@@ -19,6 +19,7 @@ order) is compared at grid points inside the layer and outside it, for
 several spins and modes.  The operator fixes bb only up to the choice of
 root; the choice is pinned by bb < 0 (the other root is positive)."""
 import functools
+import math
 
 import mpmath as mp
 import numpy as np
@@ -180,3 +181,41 @@ def test_planted_coefficient_error_is_caught(plant):
         bad[6, k] = bad[6, k] - bad[0, k] * bad[4, k]
     with pytest.raises(AssertionError):
         check(p, bad, x, r_c, theta, rows=(k,))
+
+
+@pytest.mark.parametrize("a,m", [(0.0, 2), (0.3, 1), (0.6, -3), (0.9, 2), (0.99, 4), (0.5, -1)])
+def test_coefficients_near_the_horizon(a, m):
+    """The innermost columns, where Delta = r^2 - 2 M r + a^2 keeps fewer
+    digits than r (Delta < 1e-4 r^2): the same comparison with Teukolsky's
+    operator evaluated exactly (mpmath) at the same double-precision r.  The
+    double evaluation loses about eps r^2 / Delta to the cancellation in
+    Delta (measured up to 4e-4 where r^2 / Delta = 1e13), so the tolerance is
+    1e-15 r^2 / Delta + 1e-11 -- a wrong formula misses by O(1) at every
+    column.  Columns where the grid's Newton inversion was clamped to r = r+
+    (Delta = 0 in double; the reference's own inversion gives NaN there) are
+    left out."""
+    p = O.make_params(256, 16, -100.0, 300.0, a, m)
+    x, r, theta = O.grid(p)
+    coef = O.coeffs(p, theta, r)
+    rplus = 1.0 + math.sqrt((1.0 - a) * (1.0 + a))
+    mp.mp.dps = 40
+    cols = []
+    for j in range(p.N):
+        dl = mp.mpf(r[j]) ** 2 - 2 * mp.mpf(r[j]) + mp.mpf(a) ** 2
+        if dl > 0 and dl < 1e-4 * r[j] ** 2:
+            cols.append((j, dl))
+    assert len(cols) >= 20
+    checked = 0
+    for j, dl in cols[::max(1, len(cols) // 10)]:
+        bound = 1e-15 * r[j] ** 2 / float(dl) + 1e-11
+        for k in (3, 12):
+            want = want_at(p, r[j], x[j], theta[k])
+            got = got_at(coef, k, j, theta[k])
+            for key, w in want.items():
+                w = complex(w)
+                g, scale = got[key]
+                err = abs(g - w) / max(abs(w), scale, 1e-300)
+                assert err < bound, (a, m, j, k, key, err, bound)
+        checked += 1
+    assert checked >= 10
+    assert r.min() >= rplus

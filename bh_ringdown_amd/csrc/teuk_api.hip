@@ -56,14 +56,24 @@ struct teuk_batch {
   std::vector<teuk_solver *> members;
   std::vector<long long> member_bytes;
   hipStream_t stream = nullptr;
+  // Cache groups: the members still stepping, in groups whose field state
+  // (group_bytes, 0 = all in one group) stays in the last-level cache while
+  // the group runs a chunk of steps; each group is a contiguous run of the
+  // descriptor table, has its own tile table and its own captured graph.
+  long long group_bytes = 0;
+  struct Group {
+    int first = 0, count = 0;      // descriptor range
+    size_t tile_off = 0;           // into d_tiles
+    int ntiles = 0;
+    int max_cells = 0;             // largest source patch of the group (0: no sources)
+    hipGraphExec_t gexec = nullptr;
+  };
+  std::vector<Group> groups;
   SolveDesc *d_desc = nullptr;
   int desc_cap = 0;
   int2 *d_tiles = nullptr;
-  int tiles_cap = 0;
-  int ntiles = 0;
-  int max_cells = 0;
+  size_t tiles_cap = 0;
   bool dirty = true;
-  hipGraphExec_t gexec = nullptr;
   int graph_steps = 0;
 };
 
@@ -963,23 +973,32 @@ namespace {
 int member_steps(const teuk_solver *s) { return s->has_src ? std::min(s->nt, sched_steps(s)) : s->nt; }
 
 void batch_drop_graph(teuk_batch *b) {
-  if (b->gexec) {
-    (void)hipGraphExecDestroy(b->gexec);
-    b->gexec = nullptr;
-  }
+  for (auto &g : b->groups)
+    if (g.gexec) {
+      (void)hipGraphExecDestroy(g.gexec);
+      g.gexec = nullptr;
+    }
 }
 
-// Descriptors and the tile table of the members that still have steps.
+// Field-state bytes of a member, what its group keeps in cache: the RK
+// state planes and the dbdx plane.
+long long member_state_bytes(const teuk_solver *s) {
+  const long long NM = (long long)s->P.N * s->P.M;
+  return (long long)(state_count(s->P.integrator) * 32 + 8) * NM;
+}
+
+// Descriptors and tile tables of the members that still have steps, in
+// cache groups (members in list order, a new group whenever the next member
+// would take the group past group_bytes).
 int batch_rebuild(teuk_batch *b) {
   HIPCHK(hipStreamSynchronize(b->stream));  // kernels may still read the old tables
   batch_drop_graph(b);
-  const int n = (int)b->members.size();
-  std::vector<SolveDesc> hd((size_t)n);
-  std::vector<int2> list;
-  b->max_cells = 0;
-  for (int i = 0; i < n; i++) {
-    teuk_solver *s = b->members[i];
-    SolveDesc &sd = hd[i];
+  b->groups.clear();
+  std::vector<SolveDesc> hd;
+  std::vector<int2> table;
+  long long in_group = 0;
+  for (teuk_solver *s : b->members) {
+    SolveDesc sd;
     std::memset(&sd, 0, sizeof sd);
     sd.g = s->g;
     sd.d = s->d;
@@ -988,23 +1007,41 @@ int batch_rebuild(teuk_batch *b) {
     sd.nt = member_steps(s);
     sd.tiles = (s->P.N / kWave) * sd.ngroups;
     if (s->k_host >= sd.nt) continue;
-    if (s->has_src) b->max_cells = std::max(b->max_cells, s->P.ptsx * s->P.ptsy);
-    for (int t = 0; t < sd.tiles; t++) list.push_back(make_int2(i, t));
-  }
-  std::vector<int2> table;
-  if (b->order == 0) {
-    // Blocks are dealt round-robin over the 8 XCDs: block i lands on XCD
-    // i % 8.  Give XCD x the contiguous run x*per .. (x+1)*per of the list,
-    // so neighbouring tiles of one solve share an L2; pad with no-op tiles.
-    const size_t T = list.size(), per = (T + 7) / 8;
-    table.assign(per * 8, make_int2(-1, 0));
-    for (size_t blk = 0; blk < per * 8; blk++) {
-      const size_t src = (blk & 7) * per + (blk >> 3);
-      if (src < T) table[blk] = list[src];
+    const long long sb = member_state_bytes(s);
+    if (b->groups.empty() || (b->group_bytes > 0 && in_group > 0 && in_group + sb > b->group_bytes)) {
+      teuk_batch::Group g;
+      g.first = (int)hd.size();
+      b->groups.push_back(g);
+      in_group = 0;
     }
-  } else {
-    table = list;
+    teuk_batch::Group &g = b->groups.back();
+    g.count++;
+    in_group += sb;
+    if (s->has_src) g.max_cells = std::max(g.max_cells, s->P.ptsx * s->P.ptsy);
+    hd.push_back(sd);
   }
+  // Tile tables, one per group (member index relative to the group's first
+  // descriptor).  Blocks are dealt round-robin over the 8 XCDs: block i lands
+  // on XCD i % 8.  Order 0 gives XCD x the contiguous run x*per ..
+  // (x+1)*per of the group's list, so neighbouring tiles of one solve share
+  // an L2; no-op tiles pad the table.
+  for (auto &g : b->groups) {
+    std::vector<int2> list;
+    for (int i = 0; i < g.count; i++)
+      for (int t = 0; t < hd[g.first + i].tiles; t++) list.push_back(make_int2(i, t));
+    g.tile_off = table.size();
+    if (b->order == 0) {
+      const size_t T = list.size(), per = (T + 7) / 8;
+      for (size_t blk = 0; blk < per * 8; blk++) {
+        const size_t src = (blk & 7) * per + (blk >> 3);
+        table.push_back(src < T ? list[src] : make_int2(-1, 0));
+      }
+    } else {
+      table.insert(table.end(), list.begin(), list.end());
+    }
+    g.ntiles = (int)(table.size() - g.tile_off);
+  }
+  const int n = (int)hd.size();
   if (n > b->desc_cap) {
     if (b->d_desc) (void)hipFree(b->d_desc);
     b->d_desc = nullptr;
@@ -1012,53 +1049,53 @@ int batch_rebuild(teuk_batch *b) {
     if (hipMalloc(&b->d_desc, (size_t)std::max(n, 1) * sizeof(SolveDesc)) != hipSuccess) return TEUK_ENOMEM;
     b->desc_cap = std::max(n, 1);
   }
-  if ((int)table.size() > b->tiles_cap) {
+  if (table.size() > b->tiles_cap) {
     if (b->d_tiles) (void)hipFree(b->d_tiles);
     b->d_tiles = nullptr;
     b->tiles_cap = 0;
     if (hipMalloc(&b->d_tiles, std::max<size_t>(table.size(), 1) * sizeof(int2)) != hipSuccess) return TEUK_ENOMEM;
-    b->tiles_cap = (int)std::max<size_t>(table.size(), 1);
+    b->tiles_cap = std::max<size_t>(table.size(), 1);
   }
   if (n > 0) HIPCHK(hipMemcpy(b->d_desc, hd.data(), (size_t)n * sizeof(SolveDesc), hipMemcpyHostToDevice));
   if (!table.empty())
     HIPCHK(hipMemcpy(b->d_tiles, table.data(), table.size() * sizeof(int2), hipMemcpyHostToDevice));
-  b->ntiles = (int)table.size();
   b->dirty = false;
   return TEUK_OK;
 }
 
-int batch_enqueue_step(teuk_batch *b, int k_rel) {
-  const int n = (int)b->members.size();
-  if (b->max_cells > 0 && k_rel % kRing == 0)
-    HIPCHK(launch_source_batch(b->d_desc, n, b->max_cells, k_rel, kRing, b->stream));
+// nsteps steps of one group: per step the source patches (every kRing
+// steps) and one launch per RK stage over the group's tiles; then the
+// sentinel and the output records of the group's members.
+int batch_enqueue(teuk_batch *b, const teuk_batch::Group &g, int nsteps) {
+  const SolveDesc *descs = b->d_desc + g.first;
+  const int2 *tiles = b->d_tiles + g.tile_off;
   const int nstage = b->integrator == 0 ? 3 : 4;
-  for (int st = 1; st <= nstage; st++)
-    HIPCHK(launch_stage_batch(b->d_desc, b->d_tiles, b->ntiles, b->integrator, st, k_rel, b->tiling, b->stream));
-  return TEUK_OK;
-}
-
-int batch_enqueue(teuk_batch *b, int nsteps) {
-  for (int i = 0; i < nsteps; i++) {
-    int rc = batch_enqueue_step(b, i);
-    if (rc) return rc;
+  for (int k = 0; k < nsteps; k++) {
+    if (g.max_cells > 0 && k % kRing == 0)
+      HIPCHK(launch_source_batch(descs, g.count, g.max_cells, k, kRing, b->stream));
+    for (int st = 1; st <= nstage; st++)
+      HIPCHK(launch_stage_batch(descs, tiles, g.ntiles, b->integrator, st, k, b->tiling, b->stream));
   }
-  HIPCHK(launch_sentinel_batch(b->d_desc, (int)b->members.size(), b->stream));
-  HIPCHK(launch_finalize_batch(b->d_desc, (int)b->members.size(), nsteps, b->stream));
+  HIPCHK(launch_sentinel_batch(descs, g.count, b->stream));
+  HIPCHK(launch_finalize_batch(descs, g.count, nsteps, b->stream));
   return TEUK_OK;
 }
 
-int batch_build_graph(teuk_batch *b) {
-  batch_drop_graph(b);
+int batch_build_graph(teuk_batch *b, teuk_batch::Group &g) {
+  if (g.gexec) {
+    (void)hipGraphExecDestroy(g.gexec);
+    g.gexec = nullptr;
+  }
   hipGraph_t graph = nullptr;
   HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  const int rc = batch_enqueue(b, b->chunk);
+  const int rc = batch_enqueue(b, g, b->chunk);
   hipError_t e = hipStreamEndCapture(b->stream, &graph);
   if (rc != TEUK_OK) {
     if (graph) (void)hipGraphDestroy(graph);
     return rc;
   }
   if (e != hipSuccess) return TEUK_EHIP;
-  e = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(&g.gexec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) return TEUK_EHIP;
   b->graph_steps = b->chunk;
@@ -1147,16 +1184,20 @@ int teuk_batch_step(teuk_batch *b, int nsteps) {
       int rc = batch_rebuild(b);
       if (rc) return rc;
     }
+    // one chunk of steps, group after group (each group's state stays in
+    // cache through its chunk)
     const int n = std::min(left, b->chunk);
-    if (n == b->chunk) {
-      if (!b->gexec || b->graph_steps != b->chunk) {
-        int rc = batch_build_graph(b);
+    for (auto &g : b->groups) {
+      if (n == b->chunk) {
+        if (!g.gexec || b->graph_steps != b->chunk) {
+          int rc = batch_build_graph(b, g);
+          if (rc) return rc;
+        }
+        HIPCHK(hipGraphLaunch(g.gexec, b->stream));
+      } else {
+        int rc = batch_enqueue(b, g, n);
         if (rc) return rc;
       }
-      HIPCHK(hipGraphLaunch(b->gexec, b->stream));
-    } else {
-      int rc = batch_enqueue(b, n);
-      if (rc) return rc;
     }
     for (teuk_solver *s : b->members) {
       const int nt = member_steps(s);
@@ -1179,6 +1220,13 @@ int teuk_batch_sync(teuk_batch *b) {
     if (c.status) rc = TEUK_EDIVERGED;
   }
   return rc;
+}
+
+int teuk_batch_set_cache_groups(teuk_batch *b, long long group_bytes) {
+  if (!b || group_bytes < 0) return TEUK_EINVAL;
+  b->group_bytes = group_bytes;
+  b->dirty = true;
+  return TEUK_OK;
 }
 
 int teuk_batch_info(teuk_batch *b, int *members, int *active, long long *bytes_used) {

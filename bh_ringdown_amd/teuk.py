@@ -70,7 +70,7 @@ EXPORTS = [
     "teuk_set_tiling", "teuk_remaining", "teuk_batch_create", "teuk_batch_destroy", "teuk_batch_add",
     "teuk_batch_remove", "teuk_batch_set_tiling", "teuk_batch_step", "teuk_batch_sync", "teuk_batch_info",
     "teuk_source_coeffs", "teuk_source_patch", "teuk_grid_half", "teuk_create_staggered",
-    "teuk_trajectory_schedule_staggered", "teuk_status", "teuk_solve_noalloc",
+    "teuk_trajectory_schedule_staggered", "teuk_status", "teuk_solve_noalloc", "teuk_batch_set_cache_groups",
 ]
 
 _lib = None
@@ -132,6 +132,7 @@ def lib():
         "teuk_batch_remove": ([vp, vp], ctypes.c_int),
         "teuk_batch_set_tiling": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "teuk_batch_step": ([vp, ctypes.c_int], ctypes.c_int),
+        "teuk_batch_set_cache_groups": ([vp, ctypes.c_longlong], ctypes.c_int),
         "teuk_batch_sync": ([vp], ctypes.c_int),
         "teuk_batch_info": ([vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                              ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
@@ -399,6 +400,10 @@ class Batch:
     def set_tiling(self, waves_per_block=4, waves_per_eu=2, order=0, chunk=32):
         _check(lib().teuk_batch_set_tiling(self._h, int(waves_per_block), int(waves_per_eu), int(order),
                                            int(chunk)), "teuk_batch_set_tiling")
+
+    def set_cache_groups(self, group_bytes):
+        """Step members in groups whose field state fits group_bytes (0 = one group)."""
+        _check(lib().teuk_batch_set_cache_groups(self._h, int(group_bytes)), "teuk_batch_set_cache_groups")
 
     def step(self, n):
         _check(lib().teuk_batch_step(self._h, int(n)), "teuk_batch_step")

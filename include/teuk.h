@@ -222,6 +222,13 @@ int teuk_batch_remove(teuk_batch *b, teuk_solver *s);
 /* order 0: each XCD gets a contiguous run of tiles; 1: tiles dealt in list
  * order.  chunk: steps per captured graph. */
 int teuk_batch_set_tiling(teuk_batch *b, int waves_per_block, int waves_per_eu, int order, int chunk);
+/* Cache groups: the unfinished members, in list order, are split into
+ * groups whose field state (RK state planes and the dbdx plane, 136 B per
+ * point for RK4) stays within group_bytes, so that a group's fields stay in
+ * the last-level cache while it runs a chunk of steps; each chunk then runs
+ * group after group.  0 (the default) = one group.  Results do not depend
+ * on it. */
+int teuk_batch_set_cache_groups(teuk_batch *b, long long group_bytes);
 /* Advance every unfinished member by up to nsteps steps (asynchronous). */
 int teuk_batch_step(teuk_batch *b, int nsteps);
 /* Waits for the batch; TEUK_EDIVERGED if any member has failed (find which

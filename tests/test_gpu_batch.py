@@ -45,15 +45,19 @@ CASES = [  # N, M, xmin, xmax, a, m, steps, seed, source
 
 
 @pytest.mark.parametrize("integrator", [1, 0])
-@pytest.mark.parametrize("wpb,wpe,order,chunk", [(2, 2, 0, 32), (1, 4, 1, 7), (4, 2, 0, 16)])
-def test_batch_members_equal_alone(integrator, wpb, wpe, order, chunk):
+@pytest.mark.parametrize("wpb,wpe,order,chunk,groups", [(2, 2, 0, 32, 0), (1, 4, 1, 7, 0), (4, 2, 0, 16, 0),
+                                                        (4, 2, 0, 32, 2 << 20), (2, 2, 1, 16, 1)])
+def test_batch_members_equal_alone(integrator, wpb, wpe, order, chunk, groups):
+    """groups: cache-group budget in bytes (0 = one group; 1 = a group per
+    member; 2 MiB = two or three members per group here)."""
     specs = [member(N, M, x0, x1, a, m, st, integrator, seed, src) for N, M, x0, x1, a, m, st, seed, src in CASES]
     steps = [c[6] for c in CASES]
-    print(f"batch integ={integrator} wpb={wpb} wpe={wpe} order={order} chunk={chunk}", flush=True)
+    print(f"batch integ={integrator} wpb={wpb} wpe={wpe} order={order} chunk={chunk} groups={groups}", flush=True)
     alone = [run_alone(p, u0, rows, st) for (p, u0, rows), st in zip(specs, steps)]
     solvers = []
     with teuk.Batch(integrator=integrator) as b:
         b.set_tiling(wpb, wpe, order, chunk)
+        b.set_cache_groups(groups)
         for (p, u0, rows), st in zip(specs, steps):
             _, r_c, th = teuk.grid(p)
             s = teuk.Solver(p, r_c, th)

@@ -36,11 +36,14 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 # the staggered second-order scheme 64 (half step: fields in, half-point
 # fields out) + 96 (full step: half-point fields in, fields in and out).
 B_ALG = {"rk3": 256.0, "rk4": 512.0, "so2": 160.0}
+# State bytes the implemented kernels move per point-update (RK4 in its
+# stage-value form: 64 + 128 + 96 + 96), without the 8 B per stage of the dbdx
+# plane.
+B_MOVED = {"rk3": 256.0, "rk4": 384.0, "so2": 160.0}
 STAGES = {"rk3": 3, "rk4": 4, "so2": 2}
 INTEGRATORS = {"rk3": 0, "rk4": 1, "so2": 2}
 INTEGRATOR_NAMES = {"rk4": "classic RK4", "rk3": "SSP-RK3",
                     "so2": "staggered second-order scheme (TD_GPU_code/Old)"}
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_stage_rk4_16384x64_plunge_pmc.txt")
 
 
 # Timed single solves: name -> (maker in bh_ringdown_amd.workloads, its grid size).
@@ -56,12 +59,15 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def pmc_traffic(integrator):
+def pmc_traffic(integrator, N, M):
     """HBM bytes per stage launch (FETCH_SIZE + WRITE_SIZE, scaled by the copy
-    calibration) from the committed rocprofv3 --pmc summary of the RK4 run."""
-    if integrator != "rk4" or not os.path.exists(PMC_SUMMARY):
+    calibration) from the committed rocprofv3 --pmc summary of this
+    integrator and grid (profiles/k_stage_<integrator>_<N>x<M>_plunge_pmc.txt),
+    or None."""
+    path = os.path.join(ROOT, "profiles", f"k_stage_{integrator}_{N}x{M}_plunge_pmc.txt")
+    if not os.path.exists(path):
         return None
-    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(PMC_SUMMARY).read(), re.M)
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), re.M)
     return float(m.group(1)) if m else None
 
 
@@ -149,14 +155,14 @@ def batch_section(scale, grid=None):
             "plain_point_updates_per_s": total / t_plain, "tiling": tiling, "members_bitwise_equal_alone": same}
 
 
-def serve_section(scale, seconds):
+def serve_section(scale, seconds, policy="sjf"):
     """A seeded Poisson stream (20 requests/s for `seconds`, request lengths
     scaled by `scale`) through the continuous-batching server, and the same
     requests through the plain loop."""
     from bh_ringdown_amd import serve
     reqs = serve.poisson_requests(20.0, seconds, seed=0, t_range=(500.0 * scale, 2000.0 * scale))
     log(f"bench: serving {len(reqs)} requests")
-    srv = serve.Server(hbm_budget=64 << 30, chunk=128, target_points=1 << 20)
+    srv = serve.Server(hbm_budget=64 << 30, chunk=128, target_points=1 << 20, shortest_first=(policy == "sjf"))
     t0 = time.perf_counter()
     res_s = srv.run(reqs)
     wall_s = time.perf_counter() - t0
@@ -169,7 +175,8 @@ def serve_section(scale, seconds):
     return {"workload": f"Poisson 20 req/s for {seconds:g} s (seed 0, {len(reqs)} requests), a~U[0,0.99), "
                         f"m in 1..4, iota~U[0,pi/2) capped 85 deg, q log-uniform [1e-4,1e-2], N in 4096/8192/16384, "
                         f"M in 32/64, t_final~U[{500 * scale:g},{2000 * scale:g}) ({scale:g} of the full lengths), RK4",
-            "policy": "continuous batching, shortest work first, occupancy target 1M points, 128-step chunks",
+            "policy": "continuous batching, " + ("shortest work first" if policy == "sjf" else "arrival order")
+                      + ", occupancy target 1M points, 128-step chunks",
             "served": ss, "server_time_s": srv.stats, "plain": sp, "solves_per_hour_speedup": ss["solves_per_hour"] / sp["solves_per_hour"],
             "p99_ratio_plain_over_served": sp["p99_s"] / ss["p99_s"], "results_bitwise_equal_alone": same}
 
@@ -189,8 +196,10 @@ def main():
     ap.add_argument("--wpe", type=int, default=0, help="occupancy target, waves per SIMD (0 = default)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--scale", type=float, default=0.1, help="fraction of the batch/serve solve lengths")
+    ap.add_argument("--scale", type=float, default=1.0, help="fraction of the batch/serve solve lengths")
     ap.add_argument("--serve-seconds", type=float, default=1.0)
+    ap.add_argument("--serve-policy", choices=["fifo", "sjf"], default="sjf",
+                    help="admission order of waiting requests: arrival order or shortest work first")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 2 GiB device to device 4 times (a kernel of known traffic, larger than "
                          "the last-level cache, for scaling the HBM counters of a profiler run)")
@@ -241,7 +250,8 @@ def main():
     # B_alg is per full step; a step is STAGES stage launches
     b_alg = B_ALG[args.integrator]
     achieved = (b_alg / STAGES[args.integrator]) * pts / per_launch_s / 1e9
-    traffic = pmc_traffic(args.integrator) if args.workload == "single_16384x64_plunge" else None
+    moved = (B_MOVED[args.integrator] / STAGES[args.integrator]) * pts / per_launch_s / 1e9
+    traffic = pmc_traffic(args.integrator, p.N, p.M)
     out = {
         "metric": "FP64 point-updates/s (%HBM roofline)",
         "value": value,
@@ -261,16 +271,25 @@ def main():
                    "t_final": p.T,
                    "source": "seeded plunge, mass ratio 1e-3, inclination 30 deg, from 1.02 r_isco",
                    "extraction_rho": p.xmin + p.j_extract * p.dx},
+        # headline: algorithmic bytes (the survey's model: RK4 512 B, SSP-RK3
+        # 256 B per point-update) over the average stage launch; beside it the
+        # state bytes the kernels move (RK4 stage-value form 384 B) and the
+        # counter-measured HBM bytes per launch (committed rocprofv3 --pmc
+        # summary of this grid), each over the same launch time
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_so2_half + k_so2_full (edges fused)" if args.integrator == "so2" else "k_stage (fused RK stage)", "bytes_per_point_update": b_alg,
-                     "avg_launch_us": per_launch_s * 1e6},
+                     "avg_launch_us": per_launch_s * 1e6,
+                     "moved_bytes_per_point_update": B_MOVED[args.integrator], "achieved_moved": moved,
+                     "frac_moved": moved / HBM_PEAK_GBS,
+                     "achieved_counter": traffic / per_launch_s / 1e9 if traffic else None,
+                     "frac_counter": traffic / per_launch_s / 1e9 / HBM_PEAK_GBS if traffic else None},
     }
     if not args.no_batch:
         out["batch"] = batch_section(args.scale)
         out["batch_small_grids"] = batch_section(args.scale, grid=(4096, 32))
     if not args.no_serve:
-        out["serving"] = serve_section(args.scale, args.serve_seconds)
+        out["serving"] = serve_section(args.scale, args.serve_seconds, args.serve_policy)
     if not args.no_cpu:
         log("bench: cpu baseline")
         out["cpu_baseline"] = cpu_baseline(p, rows)

@@ -52,6 +52,7 @@ def test_plunge_converges():
     ws = judged("plunge", 1e-3)
     assert len(ws) >= 2
     assert all(w["diff_24"] < w["diff_12"] and w["order"] > 0.3 for w in ws), [w["order"] for w in ws]
-    # the finest pair agrees to a few percent of the signal where it is largest
+    # where the signal is largest the finest pair still differs by 64 % of
+    # it (the coarser pair by 88 %): the sourced problem converges slowly
     top = max(ws, key=lambda w: w["rms_psi"])
-    assert top["diff_24"] < 0.7 * top["rms_psi"]
+    assert top["diff_24"] < 0.7 * top["rms_psi"] < top["diff_12"]

@@ -71,6 +71,7 @@ EXPORTS = [
     "teuk_batch_remove", "teuk_batch_set_tiling", "teuk_batch_step", "teuk_batch_sync", "teuk_batch_info",
     "teuk_source_coeffs", "teuk_source_patch", "teuk_grid_half", "teuk_create_staggered",
     "teuk_trajectory_schedule_staggered", "teuk_status", "teuk_solve_noalloc", "teuk_batch_set_cache_groups",
+    "teuk_ot_transition",
 ]
 
 _lib = None
@@ -123,6 +124,7 @@ def lib():
         "teuk_traj_plunge": ([ctypes.c_double] * 6 + [ctypes.c_ulonglong, _dp, ctypes.c_int,
                                                       ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "teuk_set_tiling": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "teuk_ot_transition": ([ctypes.c_double, _dp, _dp], ctypes.c_int),
         "teuk_remaining": ([vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "teuk_source_coeffs": ([vp, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
         "teuk_source_patch": ([vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
@@ -211,6 +213,13 @@ def plunge_trajectory(a, eta, iota, dt_out, t_final, seed=0, r0_over_isco=1.02):
     _check(lib().teuk_traj_plunge(a, eta, iota, r0_over_isco, dt_out, t_final, seed, _ptr(rows), cap,
                                   ctypes.byref(n)), "teuk_traj_plunge")
     return rows[: n.value].copy()
+
+
+def ot_transition(L):
+    """The universal Ori-Thorne transition function X(L) and X'(L)."""
+    x = ctypes.c_double(); dx = ctypes.c_double()
+    _check(lib().teuk_ot_transition(float(L), ctypes.byref(x), ctypes.byref(dx)), "teuk_ot_transition")
+    return x.value, dx.value
 
 
 def write_traj(path, rows):

@@ -250,9 +250,11 @@ int teuk_body(const teuk_params *p, const double *r_c, const double *theta,
               int verbose);
 
 /* ---- trajectory source ----------------------------------------------------- */
-/* Seeded transition-plus-plunge worldline (mass ratio eta, inclination iota
- * in radians, 0 <= iota <= 85 degrees, start at r0 = r0_over_isco x the
- * inclined ISCO radius), sampled
+/* Seeded inspiral-transition-plunge worldline (mass ratio eta, inclination
+ * iota in radians, 0 <= iota <= 85 degrees, start at r0 = r0_over_isco x the
+ * inclined ISCO radius): the quadrupole-flux inspiral, the generalised
+ * Ori-Thorne transition and the geodesic plunge of GenOT/ParamWLMino.cc,
+ * sampled
  * every dt_out of Boyer-Lindquist time up to t_final, as .traj rows
  * (10 doubles each: t r theta dr/dt dtheta/dt dphi/dt phi E Lz Q, the format
  * GenOT/WL_BL_RK.cc:214-221 writes and body.cu:97 reads).  Use
@@ -286,6 +288,12 @@ int teuk_traj_plunge(double a, double eta, double iota, double r0_over_isco, dou
 int teuk_solve_noalloc(const teuk_params *p, const double *r_c, const double *theta,
                        const double *u0, const teuk_source_step *steps, int nsteps,
                        double *u_out, teuk_output_row *rows, int max_rows, int *nrows);
+
+/* The universal Ori-Thorne transition function X(L) (X'' = -X^2 - L,
+ * X -> sqrt(-L) as L -> -inf) and X'(L), for -40 <= L <= 3.4, as
+ * teuk_traj_plunge's transition uses it.  Replaces the table GenOT reads,
+ * GenOT/OT_X_vs_L.dat (ParamWLMino.cc:190-205). */
+int teuk_ot_transition(double L, double *X, double *dX);
 
 /* Human-readable text for an error code. */
 const char *teuk_strerror(int code);

@@ -1,12 +1,19 @@
-"""Host-side plunge trajectory generator (teuk_traj_plunge): its circular
-orbits, ISCO and geodesic rows checked against closed forms and first
-integrals of Kerr geodesic motion.  No GPU is used."""
+"""Host-side trajectory generator (teuk_traj_plunge: inspiral, Ori-Thorne
+transition, plunge): its circular orbits, ISCO and geodesic rows checked
+against closed forms and first integrals of Kerr geodesic motion; its
+transition function against the reference's own table (GenOT/OT_X_vs_L.dat,
+fixture ot_x_vs_l.npz); the whole worldline against the reference's own
+plunge (QNM_Extraction/Examples/a0.7_thi000_thf090.0_n.traj, made by GenOT;
+fixture examples_a07_traj.npz).  No GPU is used."""
 import math
+import os
 
 import numpy as np
 import pytest
 
 from bh_ringdown_amd import teuk
+
+HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def kerr_isco_equatorial(a):
@@ -67,23 +74,64 @@ def test_plunge_rows_follow_geodesic_first_integrals():
     assert t[0] == 0.0
     assert np.allclose(np.diff(t), dt_out, rtol=0, atol=1e-9)
     # radiation reaction only removes energy and angular momentum
-    assert np.all(np.diff(E) <= 0) and np.all(np.diff(L) <= 0)
+    assert np.all(np.diff(E) <= 1e-15) and np.all(np.diff(L) <= 1e-14)
     assert E[-1] < E[0]
     # it plunges: ends at the horizon, radius non-increasing once inside r0 - 0.1
     rh = 1 + math.sqrt(1 - a * a)
     assert r[-1] - rh < 1e-5
     k = np.argmax(r < r[0] - 0.1)
     assert k > 0 and np.all(np.diff(r[k:]) <= 1e-12)
-    # once the constants are frozen and the velocity has relaxed onto the
-    # geodesic (a few M): (dr/dt)^2 (dt/dlambda)^2 = R(r)
-    t_freeze = t[np.argmax(E == E[-1])]
-    frozen = np.flatnonzero((E == E[-1]) & (t > t_freeze + 20.0) & (r > rh + 0.05))
+    # once the constants are frozen (the plunge) the motion is a geodesic:
+    # (dr/dl)^2 - R(r) = (dr/dt)^2 (dt/dl)^2 - R(r) is conserved (its value
+    # is set by the transition's radial speed where the plunge starts)
+    frozen = np.flatnonzero((E == E[-1]) & (r > rh + 0.05))
     assert frozen.size > 50
+    inv = []
     for i in frozen[:: max(1, frozen.size // 40)]:
         tl = dt_dlambda(r[i], th[i], a, E[i], L[i])
-        lhs = (rdot[i] * tl) ** 2
-        rhs = R(r[i], a, E[i], L[i], Q[i])
-        assert lhs == pytest.approx(rhs, rel=1e-6, abs=1e-9)
+        inv.append((rdot[i] * tl) ** 2 - R(r[i], a, E[i], L[i], Q[i]))
+    inv = np.array(inv)
+    scale = max(abs(R(r[frozen[0]], a, E[-1], L[-1], Q[-1])), np.max(np.abs(rdot[frozen] * 10.0)) ** 2, 1e-3)
+    assert np.ptp(inv) < 1e-6 * scale
+
+
+def test_transition_function_matches_genot_table():
+    """X(L), the solution of X'' = -X^2 - L with X -> sqrt(-L), against every
+    100th row of the reference's GenOT/OT_X_vs_L.dat over its whole range
+    (L = -40 .. 3.4, X = 6.3 .. -2e6): measured within 4e-5 relative."""
+    fx = np.load(os.path.join(HERE, "golden", "ot_x_vs_l.npz"))
+    for L, want in zip(fx["L"], fx["X"]):
+        x, dx = teuk.ot_transition(L)
+        assert abs(x - want) <= 1e-4 * max(1.0, abs(want)), (L, x, want)
+    # the differential equation at a few points (central differences)
+    for L in (-20.0, -3.0, 0.0, 2.35):
+        h = 1e-3
+        x0, xp, xm = teuk.ot_transition(L)[0], teuk.ot_transition(L + h)[0], teuk.ot_transition(L - h)[0]
+        assert abs((xp - 2 * x0 + xm) / h ** 2 + x0 * x0 + L) < 1e-4 * max(1.0, x0 * x0)
+    with pytest.raises(RuntimeError):
+        teuk.ot_transition(3.5)
+
+
+def test_plunge_tracks_the_reference_example():
+    """The a = 0.7 prograde equatorial worldline, mass ratio 1.3e-3 (the
+    transition's time scale, eta B, is the one free parameter: GenOT's fluxes
+    are adiabatic Teukolsky fluxes, these the quadrupole formula), aligned
+    in time where r = 3: r(t) and dphi/dt stay within 0.04 M (measured 0.026)
+    and 2 % (1.2 %) of the reference's GenOT plunge from r = 3.2 to 1.76,
+    where it is at the horizon.  The transition matters: the model this
+    generator had before (constants frozen at r_isco - 0.5, dr/dl relaxed
+    onto the geodesic) ran 5 M late over the same stretch."""
+    f = np.load(os.path.join(HERE, "golden", "examples_a07_traj.npz"))
+    rows = teuk.plunge_trajectory(0.7, 1.3e-3, 0.0, 0.01, 8000.0, seed=0, r0_over_isco=1.3)
+    t, r, phd = rows[:, 0], rows[:, 1], rows[:, 5]
+    i3, j3 = np.argmin(np.abs(r - 3.0)), np.argmin(np.abs(f["r"] - 3.0))
+    off = f["t"][j3] - t[i3]
+    sel = (f["t"] > 2441.0) & (f["r"] > 1.76)
+    assert sel.sum() > 2000
+    rg = np.interp(f["t"][sel], t + off, r)
+    pg = np.interp(f["t"][sel], t + off, phd)
+    assert np.abs(rg - f["r"][sel]).max() < 0.04
+    assert np.abs(pg / f["phidot"][sel] - 1).max() < 0.02
 
 
 def test_seed_reproducible_and_distinct():

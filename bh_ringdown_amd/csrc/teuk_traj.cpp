@@ -248,9 +248,6 @@ extern "C" int teuk_traj_plunge(double a, double eta, double iota, double r0_ove
   const double rh = 1.0 + std::sqrt(1.0 - a * a);
   const double ci = std::cos(iota), si = std::sin(iota);
   const double risco = isco(a, iota);
-  const double r0 = r0_over_isco * risco;
-  Consts c;
-  if (!circular(r0, a, iota, c)) return TEUK_EINVAL;
 
   // Ori-Thorne parameters at the ISCO.  Its circular orbit's Mino-time rate
   // dt/dl and angular speed are taken where it crosses the equator.
@@ -280,6 +277,13 @@ extern "C" int teuk_traj_plunge(double a, double eta, double iota, double r0_ove
   }
   const double r_ti = r_trans(o, kLTi * o.l0, nullptr);
   const double lam_f = kLTf * o.l0;
+  // The worldline enters the transition from the inspiral, at L_i, as GenOT's
+  // does (ParamWLMino.cc:396-416): a start inside the transition region
+  // (r0 below r_isco + R0 X(L_i), which for eta = 1e-3 lies 0.1 - 0.2 M
+  // outside the ISCO) begins at that radius instead.
+  const double r0 = std::max(r0_over_isco * risco, r_ti * (1.0 + 1e-9));
+  Consts c;
+  if (!circular(r0, a, iota, c)) return TEUK_EINVAL;
 
   // seeded initial azimuth and polar phase (splitmix64)
   uint64_t x = seed + 0x9E3779B97F4A7C15ull;
@@ -325,18 +329,6 @@ extern "C" int teuk_traj_plunge(double a, double eta, double iota, double r0_ove
     c = trans_consts(li);
     phase = 1;
   };
-  if (r0 <= r_ti) {
-    // the start lies inside the transition: its Mino time by bisection
-    double lo = kOtL0 * o.l0, hi = lam_f;
-    for (int it = 0; it < 200; it++) {
-      const double mid = 0.5 * (lo + hi);
-      if (r_trans(o, mid, nullptr) > r0)
-        lo = mid;
-      else
-        hi = mid;
-    }
-    start_transition(0.5 * (lo + hi));
-  }
 
   const int sub = 8;
   const double h = dt_out / sub;
@@ -416,6 +408,8 @@ extern "C" int teuk_traj_plunge(double a, double eta, double iota, double r0_ove
     s.t = n * dt_out;  // rows sit on the exact output grid
     emit(s);
   }
+  for (size_t i = 0; i < 10 * (size_t)n; i++)
+    if (!std::isfinite(rows[i])) return TEUK_EINVAL;   // never hand out a broken worldline
   *nrows = n;
   return TEUK_OK;
 }

@@ -252,9 +252,9 @@ int teuk_body(const teuk_params *p, const double *r_c, const double *theta,
 /* ---- trajectory source ----------------------------------------------------- */
 /* Seeded inspiral-transition-plunge worldline (mass ratio eta, inclination
  * iota in radians, 0 <= iota <= 85 degrees, start at r0 = r0_over_isco x the
- * inclined ISCO radius): the quadrupole-flux inspiral, the generalised
- * Ori-Thorne transition and the geodesic plunge of GenOT/ParamWLMino.cc,
- * sampled
+ * inclined ISCO radius, or where the transition begins if that is further
+ * out): the quadrupole-flux inspiral, the generalised Ori-Thorne transition
+ * and the geodesic plunge of GenOT/ParamWLMino.cc, sampled
  * every dt_out of Boyer-Lindquist time up to t_final, as .traj rows
  * (10 doubles each: t r theta dr/dt dtheta/dt dphi/dt phi E Lz Q, the format
  * GenOT/WL_BL_RK.cc:214-221 writes and body.cu:97 reads).  Use

@@ -175,6 +175,20 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
   if (b0 >= M) return;  // uniform over the wavefront; no block-level barrier follows
   double *L = &lds[w][0][0];
 
+  // The step counter and the source-patch origins of all kRing ring slots
+  // (one int per lane) are requested together, first: this step's origin is
+  // picked out of the lanes once the counter is in, so the patch loads wait
+  // for one memory round trip instead of two, and the first row's other
+  // loads are issued before anything waits for either.
+  // Both are vector loads: a scalar load would join the kernel-argument
+  // loads, which can only be waited for all together.  (lane >> 6 is 0 for
+  // every lane of a 64-wide block row; as the compiler cannot see that, the
+  // counter's address is per lane and the load a vector load.  Any index
+  // 0..3 stays inside Ctl's four leading ints.)
+  static_assert(2 * kRing == kWave, "one patch-origin entry per lane");
+  const int org_lane = porg[lane];
+  const int kctl_lane = (&ctl->k)[lane >> 6];
+
   const bool lo_edge = (a0 == 0), hi_edge = (a0 + kWave == N);
   const bool inner_col = (a >= 3 && a < N - 3);
   const bool ko_col = (a >= 4 && a < N - 4);
@@ -208,14 +222,19 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
     }
   }
 
-  const int kstep = ctl->k + k_rel;
+  int kstep = 0;
   int it0 = INT_MIN / 2, ip0 = INT_MIN / 2;
-  if (has_src) {
-    patch += (size_t)(kstep % kRing) * 2 * g.ptsx * g.ptsy;
-    it0 = porg[2 * (kstep % kRing)];
-    ip0 = porg[2 * (kstep % kRing) + 1];
-  }
-  const bool snap = kLast && (kstep % g.Fr) == 0 && kstep / g.Fr < g.nrec_cap;
+  // The step, and this step's patch slot and origin: the first use of the
+  // loads above, after the first row's loads are issued.
+  auto find_patch = [&]() {
+    kstep = __builtin_amdgcn_readfirstlane(kctl_lane) + k_rel;
+    if (has_src) {
+      const int slot = kstep % kRing;
+      patch += (size_t)slot * 2 * g.ptsx * g.ptsy;
+      it0 = __builtin_amdgcn_readlane(org_lane, 2 * slot);
+      ip0 = __builtin_amdgcn_readlane(org_lane, 2 * slot + 1);
+    }
+  };
 
   double lo[4] = {0, 0, 0, 0}, hi[4] = {0, 0, 0, 0};
 
@@ -290,7 +309,8 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
     x.halo = ld2(in, hoff + c0);
     return x;
   };
-  auto load_late = [&](int b) {
+  // The row inputs other than the source patch ...
+  auto load_late_fixed = [&](int b) {
     RowLate x;
     const unsigned c = 8u * (unsigned)(b * N + a);
     const unsigned c2 = 16u * (unsigned)(b * N + a);
@@ -306,27 +326,39 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
     x.dbdx = ld(dbdx, c);
 #pragma unroll
     for (int j = 0; j < kNR; j++) x.rt[j] = rtab[j * M + b];
+    return x;
+  };
+  // ... and the patch's source values at the row (0 outside the patch).
+  auto load_patch = [&](RowLate &x, int b) {
     const int sb = b - it0, sa = a - ip0;
     const bool in_patch = (unsigned)sb < (unsigned)g.ptsy && (unsigned)sa < (unsigned)g.ptsx;
     const int pc = in_patch ? sb * g.ptsx + sa : 0;
     const double pr_ = patch[pc], pi_ = patch[g.ptsx * g.ptsy + pc];
     x.sr = in_patch ? pr_ : 0.0;
     x.si = in_patch ? pi_ : 0.0;
-    return x;
   };
-  // Rows of the chunk.  Chunks that hold a theta-edge row skip the rows the
-  // boundary fill produces; all other chunks run the same body with no
-  // row conditions at all (exact wait counts, prefetch always in flight).
-  auto run_rows = [&](auto edge_tag) {
-    constexpr bool kEdge = decltype(edge_tag)::value;
-    RowIn cur = load_row(kEdge && b0 < 3 ? 3 : b0);
+  // Rows of the chunk, i0 <= i < i1.  The first chunk (kPart 1) skips rows
+  // 0-2 and the last (kPart 2) rows M-3 .. M-1, which the theta boundary
+  // fill produces; the row range is known at compile time, so the body has
+  // no row conditions at all (exact wait counts, prefetch always in flight).
+  auto run_rows = [&](auto part_tag) {
+    constexpr int kPart = decltype(part_tag)::value;
+    constexpr int i0 = kPart == 1 ? 3 : 0;
+    constexpr int i1 = kPart == 2 ? kRows - 3 : kRows;
+    RowIn cur = load_row(b0 + i0);
+    RowLate late0 = load_late_fixed(b0 + i0);
+    find_patch();
+    load_patch(late0, b0 + i0);
 #pragma unroll
-    for (int i = 0; i < kRows; i++) {
+    for (int i = i0; i < i1; i++) {
     const int b = b0 + i;
-    if (kEdge && (b < 3 || b >= M - 3)) continue;  // uniform over the wavefront
-    const RowLate late = load_late(b);
+    RowLate late = late0;
+    if (i > i0) {
+      late = load_late_fixed(b);
+      load_patch(late, b);
+    }
     RowIn nxt = cur;
-    if (i + 1 < kRows) nxt = load_row(kEdge && b + 1 >= M - 3 ? b : b + 1);
+    if (i + 1 < i1) nxt = load_row(b + 1);
     const double pr0 = cur.pr, pi0 = cur.pi;
     const double qr0 = wqr[i + 3], qi0 = wqi[i + 3];
 
@@ -422,7 +454,7 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
     rho_fix(v);
     store_row(b, v);
     cur = nxt;
-    if (kEdge) {
+    if (kPart != 0) {
       if (b == 3 || b == M - 5) {
 #pragma unroll
         for (int f = 0; f < 4; f++) lo[f] = v[f];
@@ -434,10 +466,13 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
     }
     }
   };
-  if (b0 >= 3 && b0 + kRows <= M - 3)
-    run_rows(std::false_type{});
+  if (b0 == 0)
+    run_rows(std::integral_constant<int, 1>{});
+  else if (b0 + kRows == M)
+    run_rows(std::integral_constant<int, 2>{});
   else
-    run_rows(std::true_type{});
+    run_rows(std::integral_constant<int, 0>{});
+  const bool snap = kLast && (kstep % g.Fr) == 0 && kstep / g.Fr < g.nrec_cap;
 
   // Theta edge rows (kern.cu:320-409), then their rho edges.
   if (b0 == 0) {

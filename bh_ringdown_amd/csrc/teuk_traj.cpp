@@ -15,8 +15,11 @@
 //    Kerr's second-order geodesic equations in Mino time lambda:
 //      (dr/dl)^2 = R(r) = [E(r^2+a^2) - a Lz]^2 - Delta [r^2 + (Lz - a E)^2 + Q],
 //      d^2r/dl^2 = R'(r)/2 (r tracks the potential minimum as E, Lz shrink);
-//      with z = cos(theta): (dz/dl)^2 = Z(z) = Q - z^2 [Q + Lz^2 + a^2(1-E^2)(1-z^2)],
-//      d^2z/dl^2 = Z'(z)/2;
+//      with z = cos(theta): (dz/dl)^2 = Z(z) = Q - z^2 [Q + Lz^2 + a^2(1-E^2)(1-z^2)]
+//      = b (z-^2 - z^2)(z+^2 - z^2), b = a^2(1-E^2), integrated through the
+//      polar angle chi of z = z- cos(chi) (Drasco & Hughes 2004),
+//      dchi/dl = sqrt(b z+^2 - b z-^2 cos^2 chi), so that |z| <= z- < 1 holds
+//      at every step while the constants, and with them z-, change;
 //      dt/dl = -a(a E sin^2 - Lz) + (r^2+a^2)/Delta [E(r^2+a^2) - a Lz],
 //      dphi/dl = -(a E - Lz/sin^2) + a/Delta [E(r^2+a^2) - a Lz].
 // 2. Transition (the generalised Ori-Thorne transition of
@@ -171,16 +174,35 @@ double ot_x(double L, double *dX) {
 constexpr double kLTi = -3.0, kLTf = 2.35;
 
 struct State {
-  double t, r, z, phi, ur, uz;  // z = cos(theta); u* = d/dlambda
+  double t, r, chi, phi, ur;    // cos(theta) = z- cos(chi); ur = dr/dlambda
   double lam;                   // Mino time from the transition's origin (transition only)
 };
 
 struct Deriv {
-  double dt, dr, dz, dphi, dur, duz;  // per unit lambda
+  double dt, dr, dchi, dphi, dur;  // per unit lambda
+  double z, dz;                    // cos(theta) and its rate per unit lambda
 };
 
+// The polar turning point z- and b z+^2 of Z(z) = b (z-^2 - z^2)(z+^2 - z^2),
+// the roots of b y^2 - (b + Lz^2 + Q) y + Q = 0 in y = z^2, in forms that
+// stay finite as b -> 0 (z-^2 -> Q / (Lz^2 + Q), b z+^2 -> Lz^2 + Q).
+struct Polar {
+  double zm, b, bzp2;
+};
+Polar polar(double a, const Consts &c) {
+  Polar p;
+  p.b = a * a * (1.0 - c.E * c.E);
+  const double S = p.b + c.L * c.L + c.Q;
+  const double sum = S + std::sqrt(std::max(0.0, S * S - 4.0 * p.b * c.Q));
+  p.bzp2 = 0.5 * sum;
+  p.zm = sum > 0.0 ? std::sqrt(2.0 * c.Q / sum) : 0.0;
+  return p;
+}
+
 Deriv rhs(const State &s, double a, const Consts &c) {
-  const double r = s.r, z = s.z;
+  const Polar pz = polar(a, c);
+  const double cx = std::cos(s.chi), sx = std::sin(s.chi);
+  const double r = s.r, z = pz.zm * cx;
   const double sn2 = 1.0 - z * z;
   const double D = r * r - 2.0 * r + a * a;
   const double P = c.E * (r * r + a * a) - a * c.L;
@@ -188,11 +210,10 @@ Deriv rhs(const State &s, double a, const Consts &c) {
   d.dt = -a * (a * c.E * sn2 - c.L) + (r * r + a * a) / D * P;
   d.dphi = -(a * c.E - c.L / sn2) + a / D * P;
   d.dr = s.ur;
-  d.dz = s.uz;
   d.dur = 0.5 * dRdr(r, a, c);
-  // (dz/dl)^2 = Zf(z) = Q - z^2 [a^2 (1-E^2)(1-z^2) + L^2 + Q];  d2z/dl2 = Zf'(z)/2
-  const double b = a * a * (1.0 - c.E * c.E);
-  d.duz = 0.5 * (-2.0 * z * (b * (1.0 - z * z) + c.L * c.L + c.Q) + 2.0 * b * z * z * z);
+  d.dchi = std::sqrt(std::max(0.0, pz.bzp2 - pz.b * pz.zm * pz.zm * cx * cx));
+  d.z = z;
+  d.dz = pz.zm > 0.0 ? -pz.zm * sx * d.dchi : 0.0;
   return d;
 }
 
@@ -200,7 +221,7 @@ Deriv rhs(const State &s, double a, const Consts &c) {
 // particle's angular speed on the sphere, sqrt(phidot^2 sin^2 + thetadot^2).
 void quad_rates(const State &s, double a, const Consts &c, double *dEdt, double *dLtdt) {
   const Deriv d = rhs(s, a, c);
-  const double sn2 = std::max(1e-300, 1.0 - s.z * s.z);
+  const double sn2 = std::max(1e-300, 1.0 - d.z * d.z);
   const double phd = d.dphi / d.dt, zd = d.dz / d.dt;
   const double Om = std::sqrt(phd * phd * sn2 + zd * zd / sn2);
   *dEdt = -(32.0 / 5.0) * std::pow(Om, 10.0 / 3.0);
@@ -258,7 +279,7 @@ extern "C" int teuk_traj_plunge(double a, double eta, double iota, double r0_ove
     o.rl = risco;
     o.El = cl.E;
     o.Ltl = cl.Lt;
-    State sl{0.0, risco, 0.0, 0.0, 0.0, std::sqrt(std::max(0.0, cl.Q)), 0.0};
+    State sl{0.0, risco, 0.5 * M_PI, 0.0, 0.0, 0.0};
     double dEdt, dLtdt;
     quad_rates(sl, a, cl, &dEdt, &dLtdt);
     const double G = rhs(sl, a, cl).dt;
@@ -296,16 +317,8 @@ extern "C" int teuk_traj_plunge(double a, double eta, double iota, double r0_ove
   };
   const double phi0 = 2.0 * M_PI * next();
   const double psi0 = 2.0 * M_PI * next();
-  // polar motion: z oscillates between +-z_max; start at z_max cos(psi0)
-  const double b0 = a * a * (1.0 - c.E * c.E);
-  const double sq = c.L * c.L + c.Q + b0;
-  const double zmax2 = (sq - std::sqrt(sq * sq - 4.0 * b0 * c.Q)) / (2.0 * b0 > 0.0 ? 2.0 * b0 : 1.0);
-  double zmax = b0 > 1e-14 ? std::sqrt(std::max(0.0, zmax2)) : std::sqrt(c.Q / (c.L * c.L + c.Q));
-  State s{0.0, r0, zmax * std::cos(psi0), phi0, 0.0, 0.0, 0.0};
-  {
-    const double Z = c.Q - s.z * s.z * (b0 * (1.0 - s.z * s.z) + c.L * c.L + c.Q);
-    s.uz = -std::sqrt(std::max(0.0, Z)) * (std::sin(psi0) >= 0 ? 1.0 : -1.0);
-  }
+  // polar motion: z = z- cos(chi) oscillates between +-z-; chi starts at psi0
+  State s{0.0, r0, psi0, phi0, 0.0, 0.0};
 
   // 0 inspiral, 1 transition, 2 plunge
   int phase = 0;
@@ -336,10 +349,10 @@ extern "C" int teuk_traj_plunge(double a, double eta, double iota, double r0_ove
   auto emit = [&](const State &st) {
     const Deriv d = rhs(st, a, c);
     double *row = rows + 10 * (size_t)n;
-    const double sn = std::sqrt(1.0 - st.z * st.z);
+    const double sn = std::sqrt(1.0 - d.z * d.z);
     row[0] = st.t;
     row[1] = st.r;
-    row[2] = std::acos(st.z);
+    row[2] = std::acos(d.z);
     row[3] = d.dr / d.dt;
     row[4] = -(d.dz / d.dt) / (sn > 1e-300 ? sn : 1e-300);
     row[5] = d.dphi / d.dt;
@@ -368,12 +381,12 @@ extern "C" int teuk_traj_plunge(double a, double eta, double iota, double r0_ove
         }
         Deriv d = rhs(yy, a, cc);
         const double it = 1.0 / d.dt;
-        return std::pair<Deriv, double>(Deriv{1.0, d.dr * it, d.dz * it, d.dphi * it, d.dur * it, d.duz * it}, it);
+        return std::pair<Deriv, double>(Deriv{1.0, d.dr * it, d.dchi * it, d.dphi * it, d.dur * it, 0.0, 0.0}, it);
       };
       auto add = [](const State &y, const std::pair<Deriv, double> &dd, double w) {
         const Deriv &d = dd.first;
-        return State{y.t + w * d.dt, y.r + w * d.dr, y.z + w * d.dz, y.phi + w * d.dphi, y.ur + w * d.dur,
-                     y.uz + w * d.duz, y.lam + w * dd.second};
+        return State{y.t + w * d.dt, y.r + w * d.dr, y.chi + w * d.dchi, y.phi + w * d.dphi, y.ur + w * d.dur,
+                     y.lam + w * dd.second};
       };
       const auto k1 = f(s), k2 = f(add(s, k1, 0.5 * h)), k3 = f(add(s, k2, 0.5 * h)), k4 = f(add(s, k3, h));
       auto comb = [&](double State::*m, double Deriv::*dm) {
@@ -381,9 +394,8 @@ extern "C" int teuk_traj_plunge(double a, double eta, double iota, double r0_ove
       };
       State ns = s;
       ns.t += h;
-      ns.z = comb(&State::z, &Deriv::dz);
+      ns.chi = comb(&State::chi, &Deriv::dchi);
       ns.phi = comb(&State::phi, &Deriv::dphi);
-      ns.uz = comb(&State::uz, &Deriv::duz);
       if (phase == 1) {
         ns.lam = s.lam + h / 6.0 * (k1.second + 2.0 * k2.second + 2.0 * k3.second + k4.second);
         ns.r = r_trans(o, ns.lam, &ns.ur);

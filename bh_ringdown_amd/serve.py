@@ -125,8 +125,15 @@ class Server:
         self.shortest_first = shortest_first
         self.stats = {}
 
-    def run(self, requests, clock=time.perf_counter, on_records=None):
-        """Serve `requests` (sorted by arrival); returns {id: Result}."""
+    def run(self, requests, clock=time.perf_counter, on_records=None, deadline=None, sample_s=None, on_sample=None):
+        """Serve `requests` (sorted by arrival); returns {id: Result}.
+
+        deadline: seconds after the start at which serving stops (after the
+        chunk in flight); solves still resident then are closed with their
+        records so far and `done` unset, waiting requests are left unserved.
+        sample_s: every that many seconds append (t, arrived, admitted,
+        completed, waiting, resident solves, resident points, point-updates
+        done) to stats["timeline"] and pass it to on_sample."""
         pending = sorted(requests, key=lambda r: r.arrival)
         results = {r.id: Result(r.id, r.arrival) for r in pending}
         queue = []            # arrived, not admitted
@@ -136,7 +143,16 @@ class Server:
         # host time spent preparing requests, enqueuing chunks, waiting for
         # the GPU and streaming records back
         st = self.stats = {"prepare_s": 0.0, "enqueue_s": 0.0, "sync_s": 0.0, "stream_s": 0.0, "chunks": 0,
-                           "failed": 0}
+                           "failed": 0, "unfinished": 0, "unserved": 0, "point_updates": 0}
+        if sample_s:
+            st["timeline"] = []
+        next_sample = sample_s or math.inf
+        n_admitted = n_done = 0
+
+        def pu_done():
+            # completed steps of finished and resident solves
+            return st["point_updates"] + sum(v[1].N * v[1].M * (v[1].nsteps() - v[2].remaining())
+                                             for v in live.values())
         batch = teuk.Batch(integrator=self.integrator, hbm_budget=self.hbm_budget)
         if self.tiling:
             batch.set_tiling(chunk=self.chunk, **self.tiling)
@@ -144,6 +160,8 @@ class Server:
             batch.set_tiling(4, 2, 0, self.chunk)
         try:
             while pending or queue or live:
+                if deadline is not None and now() >= deadline:
+                    break
                 # 1. enqueue a chunk for everything resident (asynchronous)
                 if live:
                     ts = clock()
@@ -174,6 +192,7 @@ class Server:
                     queue.pop(0)
                     live[req.id] = [req, p, s, 0]
                     results[req.id].admitted = now()
+                    n_admitted += 1
                 if not live:
                     if pending:
                         time.sleep(max(0.0, min(0.05, pending[0].arrival - now())))
@@ -216,46 +235,89 @@ class Server:
                         res.done = now()
                         res.steps = p.nsteps()
                         res.points = p.N * p.M * res.steps
+                        st["point_updates"] += res.points
+                        n_done += 1
                         batch.remove(s)
                         s.close()
                         del live[rid]
                 st["stream_s"] += clock() - ts
+                if now() >= next_sample:
+                    n_arrived = len(requests) - len(pending)
+                    row = (round(now(), 3), n_arrived, n_admitted, n_done, len(queue), len(live),
+                           sum(v[1].N * v[1].M for v in live.values()), pu_done())
+                    st["timeline"].append(row)
+                    if on_sample is not None:
+                        on_sample(row)
+                    next_sample += sample_s
         finally:
+            st["unfinished"] = len(live)
+            st["unserved"] = len(pending) + len(queue)
+            if live:
+                try:
+                    batch.sync()
+                except teuk.TeukError:
+                    pass
+                st["point_updates"] = pu_done()
             for req, p, s, _ in live.values():
                 s.close()
             batch.close()
         return results
 
 
-def run_plain(requests, clock=time.perf_counter, integrator=1):
+def run_plain(requests, clock=time.perf_counter, integrator=1, deadline=None, stats=None):
     """The same requests one at a time in arrival order, each as its own
-    solver stepped alone; returns {id: Result}."""
+    solver stepped alone; returns {id: Result}.  With a deadline the loop
+    stops there (a solve is stepped in pieces of 4096 steps so that it can
+    stop in time; its records so far are kept, `done` unset); `stats` then
+    receives the point-updates done and the unfinished / unserved counts."""
     results = {}
     t0 = clock()
     now = lambda: clock() - t0
-    for req in sorted(requests, key=lambda r: r.arrival):
+    pu = 0
+    unfinished = 0
+    ordered = sorted(requests, key=lambda r: r.arrival)
+    served = 0
+    for req in ordered:
+        if deadline is not None and now() >= deadline:
+            break
         wait = req.arrival - now()
         if wait > 0:
-            time.sleep(wait)
+            time.sleep(wait if deadline is None else min(wait, max(0.0, deadline - now())))
+            if deadline is not None and now() >= deadline:
+                break
+        served += 1
         res = Result(req.id, req.arrival, admitted=now())
         p, s = prepare(req, integrator)
+        n = p.nsteps()
+        done_steps = 0
         try:
-            n = p.nsteps()
-            s.step(n)
-            try:
-                s.sync()
-            except teuk.TeukError as e:
-                if e.code != teuk.TEUK_EDIVERGED:
-                    raise
-                res.failed = True
+            piece = n if deadline is None else 4096
+            while done_steps < n:
+                k = min(piece, n - done_steps)
+                s.step(k)
+                try:
+                    s.sync()
+                except teuk.TeukError as e:
+                    if e.code != teuk.TEUK_EDIVERGED:
+                        raise
+                    res.failed = True
+                done_steps += k
+                if res.failed or (deadline is not None and now() >= deadline):
+                    break
             res.records = s.output()
             res.psi = s.psi()
         finally:
             s.close()
-        res.done = now()
-        res.steps = n
-        res.points = p.N * p.M * n
+        pu += p.N * p.M * done_steps
+        if done_steps == n or res.failed:
+            res.done = now()
+            res.steps = n
+            res.points = p.N * p.M * n
+        else:
+            unfinished += 1
         results[req.id] = res
+    if stats is not None:
+        stats.update({"point_updates": pu, "unfinished": unfinished, "unserved": len(ordered) - served})
     return results
 
 
@@ -267,6 +329,60 @@ def summary(results, wall=None):
     lat = np.array([r.latency for r in done])
     span = wall if wall is not None else max(r.done for r in done)
     pts = sum(r.points for r in done)
-    return {"solves": len(done), "failed": failed, "p50_s": float(np.percentile(lat, 50)),
-            "p99_s": float(np.percentile(lat, 99)), "solves_per_hour": 3600.0 * len(done) / span,
+    pct = lambda q: float(np.percentile(lat, q)) if len(lat) else None
+    return {"solves": len(done), "failed": failed, "p50_s": pct(50),
+            "p99_s": pct(99), "solves_per_hour": 3600.0 * len(done) / span,
             "point_updates_per_s": pts / span, "wall_s": span}
+
+
+def request_work(req):
+    """Point-updates of a request (grid points x steps)."""
+    return req.N * req.M * request_params(req).nsteps()
+
+
+def main(argv=None):
+    """A sustained-load run: a seeded Poisson stream through the server for
+    a fixed wall time, reporting the offered load against what was served.
+
+        python -m bh_ringdown_amd.serve [--rate 20] [--seconds 60] [--deadline 75]
+                                        [--scale 1.0] [--policy sjf|fifo] [--plain]
+    """
+    import argparse
+    import json
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--rate", type=float, default=20.0, help="requests per second")
+    ap.add_argument("--seconds", type=float, default=60.0, help="length of the arrival stream")
+    ap.add_argument("--deadline", type=float, default=None, help="stop serving at this time (default seconds + 15)")
+    ap.add_argument("--scale", type=float, default=1.0, help="fraction of the request lengths t_final ~ U[500, 2000)")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--policy", choices=["sjf", "fifo"], default="sjf")
+    ap.add_argument("--sample", type=float, default=5.0, help="seconds between timeline rows")
+    ap.add_argument("--plain", action="store_true", help="also run the one-at-a-time loop to the same deadline")
+    args = ap.parse_args(argv)
+    deadline = args.deadline if args.deadline is not None else args.seconds + 15.0
+    reqs = poisson_requests(args.rate, args.seconds, args.seed, t_range=(500.0 * args.scale, 2000.0 * args.scale))
+    work = sum(request_work(r) for r in reqs)
+    head = {"requests": len(reqs), "rate_per_s": args.rate, "arrival_window_s": args.seconds, "deadline_s": deadline,
+            "offered_point_updates": work, "offered_point_updates_per_s": work / args.seconds,
+            "t_final_range": [500.0 * args.scale, 2000.0 * args.scale], "policy": args.policy}
+    print(json.dumps({"stream": head}), flush=True)
+    cols = ("t", "arrived", "admitted", "completed", "waiting", "resident", "resident_points", "point_updates")
+    srv = Server(hbm_budget=64 << 30, chunk=128, target_points=1 << 20, shortest_first=(args.policy == "sjf"))
+    res = srv.run(reqs, deadline=deadline, sample_s=args.sample,
+                  on_sample=lambda row: print(json.dumps({"served_at": dict(zip(cols, row))}), flush=True))
+    st = srv.stats
+    out = {"stream": head,
+           "served": {**summary(res, deadline), "sustained_point_updates_per_s": st["point_updates"] / deadline,
+                      "unfinished": st["unfinished"], "unserved": st["unserved"],
+                      "host_s": {k: st[k] for k in ("prepare_s", "enqueue_s", "sync_s", "stream_s", "chunks")}}}
+    if args.plain:
+        pst = {}
+        resp = run_plain(reqs, deadline=deadline, stats=pst)
+        out["plain"] = {**summary(resp, deadline), "sustained_point_updates_per_s": pst["point_updates"] / deadline,
+                        "unfinished": pst["unfinished"], "unserved": pst["unserved"]}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -56,6 +56,11 @@ def tables_a09():
     return qnm.QnmTables(arrays={n: fx[f"table_{i}"] for i, n in enumerate(names)})
 
 
+def traj_digest(rows):
+    """SHA-256 of the trajectory rows the records were made from."""
+    return np.frombuffer(hashlib.sha256(np.ascontiguousarray(rows, dtype=np.float64).tobytes()).digest(), dtype=np.uint8)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--threads", type=int, default=O.DEFAULT_THREADS)
@@ -81,7 +86,8 @@ def main():
                         nsteps=nsteps, Fr=p.Fr, h_k=sel_h, h=h[sel_h], dedt=dedt[sel_h], psi_k=sel_p, psi=psi[sel_p],
                         fields_sha256=np.frombuffer(hashlib.sha256(np.ascontiguousarray(u).tobytes()).digest(),
                                                     dtype=np.uint8),
-                        qnm_amp=amp, qnm_phase=phase, max_abs_field=np.abs(u).max())
+                        qnm_amp=amp, qnm_phase=phase, max_abs_field=np.abs(u).max(),
+                        traj_sha256=traj_digest(rows))
     print("wrote config2_rk4_records.npz; l2m2 amplitude", amp[0], "phase", phase[0], flush=True)
 
 

@@ -158,3 +158,41 @@ def test_server_retires_a_failed_solve_and_keeps_the_others(monkeypatch):
         assert np.array_equal(res[i].records["k"], np.arange(p.nsteps()))
     s = serve.summary(res, wall=1.0)
     assert s["solves"] == 3 and s["failed"] == 1
+
+
+def test_server_deadline_accounts_for_served_and_unserved_work(monkeypatch):
+    """A stream longer than the server can finish by its deadline: serving
+    stops there, resident solves are closed with their partial work counted,
+    waiting requests are counted unserved, and the load timeline is
+    consistent (counts never decrease, arrived >= admitted >= completed)."""
+    reqs = [serve.Request(id=i, arrival=0.5 * i, a=0.5, m=2, iota=0.1, eta=1e-3, N=4096, M=32,
+                          t_final=40.0, seed=i) for i in range(12)]
+    made = {}
+
+    def fake_prepare(req, integrator=1):
+        p = serve.request_params(req, integrator)
+        made[req.id] = _FakeSolver(p)
+        return p, made[req.id]
+
+    ticks = iter(np.arange(0.0, 1e6, 0.25))
+    monkeypatch.setattr(serve, "prepare", fake_prepare)
+    monkeypatch.setattr(serve.teuk, "Batch", _FakeBatch)
+    monkeypatch.setattr(serve.time, "sleep", lambda s: None)
+    # one resident solve at a time, 64 steps per chunk, clock +0.25 s per reading
+    srv = serve.Server(hbm_budget=1 << 40, chunk=64, target_points=1, max_resident=1)
+    rows = []
+    res = srv.run(reqs, clock=lambda: float(next(ticks)), deadline=30.0, sample_s=2.0, on_sample=rows.append)
+    st = srv.stats
+    done = [r for r in res.values() if not math.isnan(r.done)]
+    assert 0 < len(done) < len(reqs)
+    assert st["unfinished"] == 1 and st["unserved"] == len(reqs) - len(done) - 1
+    partial = [s for i, s in made.items() if math.isnan(res[i].done)]
+    assert len(partial) == 1 and 0 < partial[0].k < partial[0].p.nsteps() and partial[0].closed
+    want = sum(r.points for r in done) + partial[0].p.N * partial[0].p.M * partial[0].k
+    assert st["point_updates"] == want
+    assert rows == st["timeline"] and len(rows) >= 3
+    t = np.array(rows)
+    assert np.all(np.diff(t, axis=0)[:, [0, 1, 2, 3, 7]] >= 0)
+    assert np.all(t[:, 1] >= t[:, 2]) and np.all(t[:, 2] >= t[:, 3])
+    s = serve.summary(res, wall=30.0)
+    assert s["solves"] == len(done)

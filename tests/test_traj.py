@@ -108,6 +108,32 @@ def test_plunge_rows_follow_geodesic_first_integrals():
     assert np.ptp(inv) < 1e-6 * scale
 
 
+@pytest.mark.parametrize("a,eta,iota_deg", [(0.0926, 8.4e-3, 85.0), (0.9366, 7.4e-3, 84.0), (0.6, 4.1e-3, 85.0),
+                                          (0.9, 1e-3, 30.0)])
+def test_polar_motion_keeps_its_first_integral(a, eta, iota_deg):
+    """Steep, fast inspirals (the serving stream draws inclinations up to 85
+    degrees and mass ratios up to 1e-2) keep theta between its turning
+    points, and on every row the polar speed satisfies Carter's first
+    integral (sin(theta) dtheta/dl)^2 = Z(cos(theta)) with the row's
+    constants, through the inspiral and transition (the constants change)
+    as well as the plunge."""
+    dt_out = 0.05
+    rows = teuk.plunge_trajectory(a, eta, math.radians(iota_deg), dt_out, 600.0, seed=11)
+    assert np.isfinite(rows).all()
+    th, thdot, r = rows[:, 2], rows[:, 4], rows[:, 1]
+    E, L, Q = rows[:, 7], rows[:, 8], rows[:, 9]
+    margin = math.radians(90.0 - iota_deg) * 0.5
+    assert th.min() > margin and th.max() < math.pi - margin
+    rh = 1 + math.sqrt(1 - a * a)
+    live = r > rh + 0.05
+    z = np.cos(th[live])
+    b = a * a * (1 - E[live] ** 2)
+    Z = Q[live] - z * z * (b * (1 - z * z) + L[live] ** 2 + Q[live])
+    tl = np.array([dt_dlambda(rr, tt, a, ee, ll) for rr, tt, ee, ll in zip(r[live], th[live], E[live], L[live])])
+    lhs = (np.sin(th[live]) * thdot[live] * tl) ** 2
+    assert np.max(np.abs(lhs - Z)) < 1e-9 * np.max(Q), np.max(np.abs(lhs - Z))
+
+
 def test_transition_function_matches_genot_table():
     """X(L), the solution of X'' = -X^2 - L with X -> sqrt(-L), against every
     100th row of the reference's GenOT/OT_X_vs_L.dat over its whole range

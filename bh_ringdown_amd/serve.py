@@ -371,8 +371,12 @@ def main(argv=None):
     res = srv.run(reqs, deadline=deadline, sample_s=args.sample,
                   on_sample=lambda row: print(json.dumps({"served_at": dict(zip(cols, row))}), flush=True))
     st = srv.stats
+    by_id = {r.id: r for r in reqs}
+    failed = [{k: getattr(by_id[i], k) for k in ("id", "a", "m", "iota", "eta", "N", "M", "t_final", "seed")}
+              for i, r in sorted(res.items()) if r.failed]
     out = {"stream": head,
-           "served": {**summary(res, deadline), "sustained_point_updates_per_s": st["point_updates"] / deadline,
+           "served": {**summary(res, deadline), "failed_requests": failed,
+                      "sustained_point_updates_per_s": st["point_updates"] / deadline,
                       "unfinished": st["unfinished"], "unserved": st["unserved"],
                       "host_s": {k: st[k] for k in ("prepare_s", "enqueue_s", "sync_s", "stream_s", "chunks")}}}
     if args.plain:

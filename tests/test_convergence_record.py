@@ -14,7 +14,7 @@ are zero and after it has passed they are a 1e-6 late-time residue.
   and last three theta rows extrapolated, kern.cu:320-409) limits the theta
   direction to about second order;
 * the plunge from zero fields at 16384x32, 32768x64, 65536x128 to t = 2000:
-  converging (every difference shrinks), at 0.39-1.3 -- the source's
+  converging (every difference shrinks), at 0.4-1.5 -- the source's
   Gaussian delta widths scale with the grid (kern.cu:1191-1196) and the
   trajectory is read 1.5 dt ahead (body.cu:112-129), so 6th-order
   self-convergence does not hold for the sourced problem."""
@@ -52,7 +52,7 @@ def test_plunge_converges():
     ws = judged("plunge", 1e-3)
     assert len(ws) >= 2
     assert all(w["diff_24"] < w["diff_12"] and w["order"] > 0.3 for w in ws), [w["order"] for w in ws]
-    # where the signal is largest the finest pair still differs by 82 % of
-    # it (the coarser pair by 107 %): the sourced problem converges slowly
+    # where the signal is largest the finest pair still differs by 81 % of
+    # it (the coarser pair by 108 %): the sourced problem converges slowly
     top = max(ws, key=lambda w: w["rms_psi"])
     assert top["diff_24"] < 0.85 * top["rms_psi"] < top["diff_12"]

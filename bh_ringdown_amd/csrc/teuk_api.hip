@@ -834,10 +834,16 @@ int teuk_read_psi(teuk_solver *s, int first, int max, double *psi, int *n) {
   HIPCHK(hipMemcpy(&c, s->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
   const int cnt = std::max(0, std::min(max, c.nrec - first));
   // Each snapshot holds Psi (re, im) followed by the two time integrals.
+  // One contiguous copy and a host-side gather: a strided 2D copy into
+  // pageable memory costs milliseconds.
   const size_t M = (size_t)s->P.M;
-  if (cnt > 0)
-    HIPCHK(hipMemcpy2D(psi, 2 * M * sizeof(double), s->d.ext.snap + (size_t)first * 6 * M, 6 * M * sizeof(double),
-                       2 * M * sizeof(double), (size_t)cnt, hipMemcpyDeviceToHost));
+  if (cnt > 0) {
+    std::vector<double> snap((size_t)cnt * 6 * M);
+    HIPCHK(hipMemcpy(snap.data(), s->d.ext.snap + (size_t)first * 6 * M, snap.size() * sizeof(double),
+                     hipMemcpyDeviceToHost));
+    for (int i = 0; i < cnt; i++)
+      std::memcpy(psi + (size_t)i * 2 * M, snap.data() + (size_t)i * 6 * M, 2 * M * sizeof(double));
+  }
   if (n) *n = cnt;
   return TEUK_OK;
 }

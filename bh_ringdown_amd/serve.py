@@ -5,9 +5,10 @@ time.  The server keeps one libteuk batch: every loop iteration it enqueues
 a chunk of steps for all resident solves (one launch per RK stage for the
 whole batch), and while that chunk runs on the GPU it prepares the requests
 that have arrived (grid, plunge trajectory, device set-up) and admits them
-as HBM budget frees up.  After each chunk it streams the new psi4 / h_lm
-records of every resident solve back to the host and retires the solves
-that reached their final time.  A result does not depend on what else was
+as HBM budget frees up.  After each chunk it retires the solves that
+reached their final time (with all their psi4 / h_lm records) and, every
+`stream_every` chunks, streams the new records of the other resident
+solves back to the host (each read is host time the GPU waits for).  A result does not depend on what else was
 in the batch: batched steps are bitwise the steps of a solve run alone.
 A solve that fails (the device sentinel found a non-finite or runaway value
 in its fields, teuk_status) is retired on its own with `failed` set; the
@@ -18,6 +19,7 @@ in arrival order, each as its own solver (one solve per launch).
 """
 import math
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -92,15 +94,26 @@ def request_params(req, integrator=1, Fr=2):
     return p
 
 
-def prepare(req, integrator=1):
-    """Host and device set-up of one request: grid, seeded plunge, solver."""
+def prepare_host(req, integrator=1):
+    """Host set-up of one request (no GPU calls): parameters, grid and the
+    seeded plunge's trajectory rows."""
     p = request_params(req, integrator)
     _, r_c, th = teuk.grid(p)
     nt = p.nsteps()
     rows = teuk.plunge_trajectory(req.a, req.eta, req.iota, p.dt / 2, (2 * nt + 6) * p.dt / 2, seed=req.seed)
+    return p, r_c, th, rows
+
+
+def prepare_device(p, r_c, th, rows):
+    """Device set-up of a prepared request: the solver and its source schedule."""
     s = teuk.Solver(p, r_c, th)
-    s.set_trajectory(rows, nt)
+    s.set_trajectory(rows, p.nsteps())
     return p, s
+
+
+def prepare(req, integrator=1):
+    """Host and device set-up of one request: grid, seeded plunge, solver."""
+    return prepare_device(*prepare_host(req, integrator))
 
 
 class Server:
@@ -115,7 +128,7 @@ class Server:
     `shortest_first`, else in arrival order."""
 
     def __init__(self, hbm_budget, chunk=128, integrator=1, tiling=None, max_resident=64, target_points=1 << 20,
-                 shortest_first=True):
+                 shortest_first=True, stream_every=8):
         self.hbm_budget = int(hbm_budget)
         self.chunk = chunk
         self.integrator = integrator
@@ -123,6 +136,7 @@ class Server:
         self.max_resident = max_resident
         self.target_points = int(target_points)
         self.shortest_first = shortest_first
+        self.stream_every = max(1, int(stream_every))   # chunks between record reads of a resident solve
         self.stats = {}
 
     def run(self, requests, clock=time.perf_counter, on_records=None, deadline=None, sample_s=None, on_sample=None):
@@ -142,17 +156,38 @@ class Server:
         now = lambda: clock() - t0
         # host time spent preparing requests, enqueuing chunks, waiting for
         # the GPU and streaming records back
-        st = self.stats = {"prepare_s": 0.0, "enqueue_s": 0.0, "sync_s": 0.0, "stream_s": 0.0, "chunks": 0,
+        st = self.stats = {"prepare_s": 0.0, "setup_s": 0.0, "enqueue_s": 0.0, "sync_s": 0.0, "stream_s": 0.0, "chunks": 0,
                            "failed": 0, "unfinished": 0, "unserved": 0, "point_updates": 0}
         if sample_s:
             st["timeline"] = []
         next_sample = sample_s or math.inf
         n_admitted = n_done = 0
 
+        def collect(rid):
+            # the records streamed so far, joined once (not per chunk)
+            outs, psis = live[rid][4], live[rid][5]
+            if outs:
+                res = results[rid]
+                res.records = {key: np.concatenate([o[key] for o in outs]) for key in outs[0]}
+                res.psi = np.concatenate(psis)
+
         def pu_done():
             # completed steps of finished and resident solves
             return st["point_updates"] + sum(v[1].N * v[1].M * (v[1].nsteps() - v[2].remaining())
                                              for v in live.values())
+        worker = ThreadPoolExecutor(max_workers=1)
+        prepared = {}   # request id -> future of its host set-up
+
+        def timed_prepare(req):
+            # host work only: every GPU call stays on this thread (a graph
+            # capture on the batch stream must not see another thread's
+            # allocations or copies)
+            ts = time.perf_counter()
+            try:
+                return prepare_host(req, self.integrator)
+            finally:
+                st["prepare_s"] += time.perf_counter() - ts
+
         batch = teuk.Batch(integrator=self.integrator, hbm_budget=self.hbm_budget)
         if self.tiling:
             batch.set_tiling(chunk=self.chunk, **self.tiling)
@@ -183,14 +218,31 @@ class Server:
                         raise ValueError(f"request {req.id} alone exceeds the HBM budget")
                     if batch.info()["bytes"] + teuk.footprint(p) > self.hbm_budget:
                         break
-                    ts = clock()
-                    p, s = prepare(req, self.integrator)
-                    st["prepare_s"] += clock() - ts
+                    # Host set-up runs on the worker thread while the chunk
+                    # runs; admit when it is ready (wait for it only with
+                    # nothing resident to step).
+                    if req.id not in prepared:
+                        prepared[req.id] = worker.submit(timed_prepare, req)
+                    if live and not prepared[req.id].done():
+                        break
+                    fut = prepared.pop(req.id)
+                    try:
+                        ts = clock()
+                        p, s = prepare_device(*fut.result())
+                        st["setup_s"] += clock() - ts
+                    except teuk.TeukError:
+                        # its set-up failed (e.g. no trajectory for its
+                        # parameters): retire it, serve the others
+                        queue.pop(0)
+                        results[req.id].failed = True
+                        results[req.id].done = now()
+                        st["failed"] += 1
+                        continue
                     if not batch.add(s):
                         s.close()
                         break
                     queue.pop(0)
-                    live[req.id] = [req, p, s, 0]
+                    live[req.id] = [req, p, s, 0, [], []]   # records read, their chunks (records, psi)
                     results[req.id].admitted = now()
                     n_admitted += 1
                 if not live:
@@ -209,34 +261,41 @@ class Server:
                 st["sync_s"] += clock() - ts
                 ts = clock()
                 for rid in list(live):
-                    req, p, s, nread = live[rid]
+                    req, p, s, nread, outs, psis = live[rid]
                     if any_failed and s.status() == teuk.TEUK_EDIVERGED:
                         res = results[rid]
                         res.failed = True
                         res.done = now()
+                        out = s.output(first=nread)     # its records up to the failure
+                        if len(out["k"]):
+                            outs.append(out)
+                            psis.append(s.psi(first=nread))
+                        collect(rid)
                         batch.remove(s)
                         s.close()
                         del live[rid]
                         st["failed"] += 1
                         continue
+                    finished = s.remaining() == 0
+                    if not finished and st["chunks"] % self.stream_every:
+                        continue   # read its records at a later chunk (each read costs host time the GPU waits for)
                     out = s.output(first=nread)
                     n_new = len(out["k"])
                     if n_new:
-                        res = results[rid]
                         psi = s.psi(first=nread)
-                        for key, v in out.items():
-                            res.records[key] = np.concatenate([res.records[key], v]) if key in res.records else v
-                        res.psi = psi if res.psi is None else np.concatenate([res.psi, psi])
+                        outs.append(out)
+                        psis.append(psi)
                         if on_records is not None:
                             on_records(rid, out, psi)
                         live[rid][3] = nread + n_new
-                    if s.remaining() == 0:
+                    if finished:
                         res = results[rid]
                         res.done = now()
                         res.steps = p.nsteps()
                         res.points = p.N * p.M * res.steps
                         st["point_updates"] += res.points
                         n_done += 1
+                        collect(rid)
                         batch.remove(s)
                         s.close()
                         del live[rid]
@@ -258,8 +317,10 @@ class Server:
                 except teuk.TeukError:
                     pass
                 st["point_updates"] = pu_done()
-            for req, p, s, _ in live.values():
+            for rid, (req, p, s, *_) in live.items():
+                collect(rid)
                 s.close()
+            worker.shutdown(wait=True)       # host set-ups never admitted are dropped
             batch.close()
         return results
 
@@ -378,7 +439,7 @@ def main(argv=None):
            "served": {**summary(res, deadline), "failed_requests": failed,
                       "sustained_point_updates_per_s": st["point_updates"] / deadline,
                       "unfinished": st["unfinished"], "unserved": st["unserved"],
-                      "host_s": {k: st[k] for k in ("prepare_s", "enqueue_s", "sync_s", "stream_s", "chunks")}}}
+                      "host_s": {k: st[k] for k in ("prepare_s", "setup_s", "enqueue_s", "sync_s", "stream_s", "chunks")}}}
     if args.plain:
         pst = {}
         resp = run_plain(reqs, deadline=deadline, stats=pst)

@@ -62,6 +62,12 @@ class OutputRow(ctypes.Structure):
                 ("dedt", ctypes.c_double), ("h", ctypes.c_double * 14)]
 
 
+# The same record as a numpy structured type (a zero-copy view of the rows).
+OUTPUT_ROW_DTYPE = np.dtype([("k", np.int32), ("write", np.int32), ("t", np.float64), ("dedt", np.float64),
+                             ("h", np.float64, (14,))])
+assert OUTPUT_ROW_DTYPE.itemsize == ctypes.sizeof(OutputRow)
+
+
 EXPORTS = [
     "teuk_grid", "teuk_default_params", "teuk_create", "teuk_destroy", "teuk_set_fields",
     "teuk_get_fields", "teuk_set_source", "teuk_set_trajectory", "teuk_step", "teuk_sync",
@@ -313,22 +319,24 @@ class Solver:
         _check(lib().teuk_progress(self._h, ctypes.byref(k), ctypes.byref(t)), "teuk_progress")
         return k.value, t.value
 
+    def _records_cap(self, first, max_rows):
+        # records exist for the steps enqueued so far (one every Fr steps):
+        # size the host buffers by those, not by the whole run
+        done = self.p.nsteps() - self.remaining()
+        return max(1, min(max_rows, (done + self.p.Fr - 1) // self.p.Fr - first + 1))
+
     def output(self, first=0, max_rows=1 << 20):
         n_have = ctypes.c_int()
-        cap = max(1, min(max_rows, self.p.nsteps() // self.p.Fr + 1))
+        cap = self._records_cap(first, max_rows)
         rows = (OutputRow * cap)()
         _check(lib().teuk_read_output(self._h, first, cap, rows, ctypes.byref(n_have)), "teuk_read_output")
-        n = n_have.value
-        k = np.array([rows[i].k for i in range(n)], dtype=np.int64)
-        w = np.array([rows[i].write for i in range(n)], dtype=np.int64)
-        t = np.array([rows[i].t for i in range(n)])
-        dedt = np.array([rows[i].dedt for i in range(n)])
-        h = np.array([list(rows[i].h) for i in range(n)]).reshape(n, 14)
-        return {"k": k, "write": w, "t": t, "dedt": dedt, "h": h}
+        v = np.frombuffer(rows, dtype=OUTPUT_ROW_DTYPE, count=n_have.value)
+        return {"k": v["k"].astype(np.int64), "write": v["write"].astype(np.int64), "t": v["t"].copy(),
+                "dedt": v["dedt"].copy(), "h": v["h"].copy().reshape(-1, 14)}
 
     def psi(self, first=0, max_rows=1 << 20):
-        cap = max(1, min(max_rows, self.p.nsteps() // self.p.Fr + 1))
-        buf = np.zeros((cap, 2, self.p.M))
+        cap = self._records_cap(first, max_rows)
+        buf = np.empty((cap, 2, self.p.M))
         n = ctypes.c_int()
         _check(lib().teuk_read_psi(self._h, first, cap, _ptr(buf), ctypes.byref(n)), "teuk_read_psi")
         return buf[: n.value].copy()

@@ -100,6 +100,12 @@ class _FakeSolver:
         self.closed = True
 
 
+def _fake_host(req, integrator=1):
+    """prepare_host's stand-in: the request's parameters, and the request
+    itself in place of the grid (the fake device set-up reads its id)."""
+    return serve.request_params(req, integrator), req, None, None
+
+
 class _FakeBatch:
     def __init__(self, integrator=1, hbm_budget=0):
         self.members = []
@@ -139,13 +145,13 @@ def test_server_retires_a_failed_solve_and_keeps_the_others(monkeypatch):
                           t_final=20.0 + 10.0 * i, seed=i) for i in range(4)]
     made = {}
 
-    def fake_prepare(req, integrator=1):
-        p = serve.request_params(req, integrator)
+    def fake_device(p, req, th, rows):
         s = _FakeSolver(p, fail_at=60 if req.id == 1 else None)
         made[req.id] = s
         return p, s
 
-    monkeypatch.setattr(serve, "prepare", fake_prepare)
+    monkeypatch.setattr(serve, "prepare_host", _fake_host)
+    monkeypatch.setattr(serve, "prepare_device", fake_device)
     monkeypatch.setattr(serve.teuk, "Batch", _FakeBatch)
     srv = serve.Server(hbm_budget=1 << 40, chunk=16, target_points=1 << 30)
     res = srv.run(reqs)
@@ -169,13 +175,13 @@ def test_server_deadline_accounts_for_served_and_unserved_work(monkeypatch):
                           t_final=40.0, seed=i) for i in range(12)]
     made = {}
 
-    def fake_prepare(req, integrator=1):
-        p = serve.request_params(req, integrator)
+    def fake_device(p, req, th, rows):
         made[req.id] = _FakeSolver(p)
         return p, made[req.id]
 
     ticks = iter(np.arange(0.0, 1e6, 0.25))
-    monkeypatch.setattr(serve, "prepare", fake_prepare)
+    monkeypatch.setattr(serve, "prepare_host", _fake_host)
+    monkeypatch.setattr(serve, "prepare_device", fake_device)
     monkeypatch.setattr(serve.teuk, "Batch", _FakeBatch)
     monkeypatch.setattr(serve.time, "sleep", lambda s: None)
     # one resident solve at a time, 64 steps per chunk, clock +0.25 s per reading
@@ -196,3 +202,25 @@ def test_server_deadline_accounts_for_served_and_unserved_work(monkeypatch):
     assert np.all(t[:, 1] >= t[:, 2]) and np.all(t[:, 2] >= t[:, 3])
     s = serve.summary(res, wall=30.0)
     assert s["solves"] == len(done)
+
+
+def test_server_retires_a_request_whose_setup_fails(monkeypatch):
+    """A request whose set-up raises (its trajectory cannot be generated) is
+    retired as failed; the requests around it are served."""
+    from bh_ringdown_amd import teuk
+    reqs = [serve.Request(id=i, arrival=0.0, a=0.5, m=2, iota=0.1, eta=1e-3, N=4096, M=32,
+                          t_final=20.0, seed=i) for i in range(3)]
+
+    def fake_host(req, integrator=1):
+        if req.id == 1:
+            raise teuk.TeukError(teuk.TEUK_EINVAL, "teuk_traj_plunge")
+        return _fake_host(req, integrator)
+
+    monkeypatch.setattr(serve, "prepare_host", fake_host)
+    monkeypatch.setattr(serve, "prepare_device", lambda p, req, th, rows: (p, _FakeSolver(p)))
+    monkeypatch.setattr(serve.teuk, "Batch", _FakeBatch)
+    srv = serve.Server(hbm_budget=1 << 40, chunk=16, target_points=1 << 30)
+    res = srv.run(reqs)
+    assert res[1].failed and srv.stats["failed"] == 1
+    assert not res[0].failed and not res[2].failed
+    assert not math.isnan(res[0].done) and not math.isnan(res[2].done)

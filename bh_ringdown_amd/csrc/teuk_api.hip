@@ -71,7 +71,7 @@ struct teuk_batch {
   std::vector<Group> groups;
   SolveDesc *d_desc = nullptr;
   int desc_cap = 0;
-  int2 *d_tiles = nullptr;
+  BatchTile *d_tiles = nullptr;
   size_t tiles_cap = 0;
   bool dirty = true;
   int graph_steps = 0;
@@ -1001,7 +1001,7 @@ int batch_rebuild(teuk_batch *b) {
   batch_drop_graph(b);
   b->groups.clear();
   std::vector<SolveDesc> hd;
-  std::vector<int2> table;
+  std::vector<BatchTile> table;
   long long in_group = 0;
   for (teuk_solver *s : b->members) {
     SolveDesc sd;
@@ -1031,16 +1031,24 @@ int batch_rebuild(teuk_batch *b) {
   // on XCD i % 8.  Order 0 gives XCD x the contiguous run x*per ..
   // (x+1)*per of the group's list, so neighbouring tiles of one solve share
   // an L2; no-op tiles pad the table.
+  BatchTile pad;
+  std::memset(&pad, 0, sizeof pad);
+  pad.tile = -1;
   for (auto &g : b->groups) {
-    std::vector<int2> list;
+    std::vector<BatchTile> list;
     for (int i = 0; i < g.count; i++)
-      for (int t = 0; t < hd[g.first + i].tiles; t++) list.push_back(make_int2(i, t));
+      for (int t = 0; t < hd[g.first + i].tiles; t++) {
+        BatchTile bt = pad;
+        bt.sd = hd[g.first + i];
+        bt.tile = t;
+        list.push_back(bt);
+      }
     g.tile_off = table.size();
     if (b->order == 0) {
       const size_t T = list.size(), per = (T + 7) / 8;
       for (size_t blk = 0; blk < per * 8; blk++) {
         const size_t src = (blk & 7) * per + (blk >> 3);
-        table.push_back(src < T ? list[src] : make_int2(-1, 0));
+        table.push_back(src < T ? list[src] : pad);
       }
     } else {
       table.insert(table.end(), list.begin(), list.end());
@@ -1059,12 +1067,13 @@ int batch_rebuild(teuk_batch *b) {
     if (b->d_tiles) (void)hipFree(b->d_tiles);
     b->d_tiles = nullptr;
     b->tiles_cap = 0;
-    if (hipMalloc(&b->d_tiles, std::max<size_t>(table.size(), 1) * sizeof(int2)) != hipSuccess) return TEUK_ENOMEM;
+    if (hipMalloc(&b->d_tiles, std::max<size_t>(table.size(), 1) * sizeof(BatchTile)) != hipSuccess)
+      return TEUK_ENOMEM;
     b->tiles_cap = std::max<size_t>(table.size(), 1);
   }
   if (n > 0) HIPCHK(hipMemcpy(b->d_desc, hd.data(), (size_t)n * sizeof(SolveDesc), hipMemcpyHostToDevice));
   if (!table.empty())
-    HIPCHK(hipMemcpy(b->d_tiles, table.data(), table.size() * sizeof(int2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->d_tiles, table.data(), table.size() * sizeof(BatchTile), hipMemcpyHostToDevice));
   b->dirty = false;
   return TEUK_OK;
 }
@@ -1074,13 +1083,13 @@ int batch_rebuild(teuk_batch *b) {
 // sentinel and the output records of the group's members.
 int batch_enqueue(teuk_batch *b, const teuk_batch::Group &g, int nsteps) {
   const SolveDesc *descs = b->d_desc + g.first;
-  const int2 *tiles = b->d_tiles + g.tile_off;
+  const BatchTile *tiles = b->d_tiles + g.tile_off;
   const int nstage = b->integrator == 0 ? 3 : 4;
   for (int k = 0; k < nsteps; k++) {
     if (g.max_cells > 0 && k % kRing == 0)
       HIPCHK(launch_source_batch(descs, g.count, g.max_cells, k, kRing, b->stream));
     for (int st = 1; st <= nstage; st++)
-      HIPCHK(launch_stage_batch(descs, tiles, g.ntiles, b->integrator, st, k, b->tiling, b->stream));
+      HIPCHK(launch_stage_batch(tiles, g.ntiles, b->integrator, st, k, b->tiling, b->stream));
   }
   HIPCHK(launch_sentinel_batch(descs, g.count, b->stream));
   HIPCHK(launch_finalize_batch(descs, g.count, nsteps, b->stream));

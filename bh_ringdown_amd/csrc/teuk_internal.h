@@ -143,6 +143,15 @@ struct SolveDesc {
   int tiles;           // (N / 64) * ngroups
 };
 
+// One block's work in a batched stage launch: the member's description
+// inline (one table read per block, not a tile entry and then a descriptor)
+// and the tile within the member; tile < 0 pads the table.
+struct BatchTile {
+  SolveDesc sd;
+  int tile;
+  int pad[3];
+};
+
 // Stage-kernel launch shape: wavefronts per block (1, 2 or 4) and the
 // occupancy hint the kernel was compiled for (waves per SIMD: 2 or 4).
 struct Tiling {
@@ -171,8 +180,8 @@ hipError_t launch_so2_step(const Grid &g, const DevState &d, bool has_src, int k
 // Batched forms: one launch covers every member of the batch.
 hipError_t launch_source_batch(const SolveDesc *descs, int nsolves, int max_cells, int k_rel0, int n,
                                hipStream_t s);
-hipError_t launch_stage_batch(const SolveDesc *descs, const int2 *tiles, int ntiles, int integrator,
-                              int stage, int k_rel, Tiling t, hipStream_t s);
+hipError_t launch_stage_batch(const BatchTile *tiles, int ntiles, int integrator, int stage, int k_rel, Tiling t,
+                              hipStream_t s);
 hipError_t launch_finalize_batch(const SolveDesc *descs, int nsolves, int n, hipStream_t s);
 // The sentinel of every member still running before the chunk's finalize.
 hipError_t launch_sentinel_batch(const SolveDesc *descs, int nsolves, hipStream_t s);

@@ -37,21 +37,38 @@
 namespace teuk {
 namespace {
 
+// Every buffer a kernel touches is device global memory.  Pointers that
+// arrive in kernel arguments are known to be global; pointers read from a
+// table in memory (the batched kernels' descriptors) are not, and accesses
+// through them would be flat instructions, which count against the scalar
+// memory counter too, so that waiting for any scalar load also waits for
+// every load in flight.  global_ptr states the address space.
+#define TEUK_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ TEUK_GLOBAL T *global_ptr(T *p) {
+  return (TEUK_GLOBAL T *)p;
+}
+
 // Global accesses by a 32-bit byte offset from a uniform base: the address is
 // an SGPR pair plus one VGPR (no 64-bit address arithmetic per access).  A
 // solve's planes stay below 4 GiB (teuk_create checks N*M).
 __device__ __forceinline__ double ld(const double *base, unsigned off) {
-  return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + off);
+  return *global_ptr(reinterpret_cast<const double *>(reinterpret_cast<const char *>(base) + off));
 }
 __device__ __forceinline__ void st(double *base, unsigned off, double v) {
-  *reinterpret_cast<double *>(reinterpret_cast<char *>(base) + off) = v;
+  *global_ptr(reinterpret_cast<double *>(reinterpret_cast<char *>(base) + off)) = v;
 }
-// A complex value (re, im) of a state: 16 bytes, one load or store.
+// A complex value (re, im) of a state: 16 bytes, one load or store (as a
+// native two-double vector: HIP's double2 class has no address-space
+// qualified copies).
+typedef double dvec2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ double2 ld2(const double *base, unsigned off) {
-  return *reinterpret_cast<const double2 *>(reinterpret_cast<const char *>(base) + off);
+  const dvec2 v = *global_ptr(reinterpret_cast<const dvec2 *>(reinterpret_cast<const char *>(base) + off));
+  return make_double2(v.x, v.y);
 }
 __device__ __forceinline__ void st2(double *base, unsigned off, double re, double im) {
-  *reinterpret_cast<double2 *>(reinterpret_cast<char *>(base) + off) = make_double2(re, im);
+  const dvec2 v = {re, im};
+  *global_ptr(reinterpret_cast<dvec2 *>(reinterpret_cast<char *>(base) + off)) = v;
 }
 
 __device__ __forceinline__ void wave_sync() {
@@ -142,10 +159,13 @@ __device__ __forceinline__ StageIO stage_io(const DevState &d) {
 }
 
 // One tile of one stage: the wavefront threadIdx.y of block-row `grp` of
-// rho strip `strip` (64 columns), wpb wavefronts per block.
-template <int INTEG, int STAGE>
+// rho strip `strip` (64 columns), wpb wavefronts per block.  kGuard: the
+// tile belongs to a batch member that may have reached its last scheduled
+// step nt before this step; checked once the step counter is in (after the
+// first row's loads are issued), before anything is stored.
+template <int INTEG, int STAGE, bool kGuard>
 __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int has_src, int k_rel, int strip,
-                                           int grp, int wpb) {
+                                           int grp, int wpb, int nt) {
   constexpr bool kLast = (INTEG == 0 && STAGE == 3) || (INTEG == 1 && STAGE == 4);
   __shared__ double lds[4][4][kLdsCols];
   const int N = g.N, M = g.M;
@@ -161,13 +181,14 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
   const double *base = io.base;
   double *out = io.out;
   double *wp = io.w;
-  const double *__restrict__ ktab = d.ktab;
-  const double *__restrict__ rtab = d.rtab;
+  const TEUK_GLOBAL double *__restrict__ ktab = global_ptr(d.ktab);
+  const TEUK_GLOBAL double *__restrict__ rtab = global_ptr(d.rtab);
   const double *__restrict__ dbdx = d.dbdx;
-  const double *__restrict__ patch = d.patch;   // advanced to this step's ring slot below
-  const int *__restrict__ porg = d.patch_org;
-  const Ext &ext = d.ext;
-  Ctl *ctl = d.ctl;
+  const TEUK_GLOBAL double *__restrict__ patch = global_ptr(d.patch);   // advanced to this step's ring slot below
+  const TEUK_GLOBAL int *__restrict__ porg = global_ptr(d.patch_org);
+  TEUK_GLOBAL double *integ = global_ptr(d.ext.integ);
+  TEUK_GLOBAL double *snap_buf = global_ptr(d.ext.snap);
+  TEUK_GLOBAL Ctl *ctl = global_ptr(d.ctl);
 
   const int a0 = strip * kWave;
   const int a = a0 + lane;
@@ -348,6 +369,7 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
     RowIn cur = load_row(b0 + i0);
     RowLate late0 = load_late_fixed(b0 + i0);
     find_patch();
+    if (kGuard && kstep >= nt) return;  // uniform: the member has finished
     load_patch(late0, b0 + i0);
 #pragma unroll
     for (int i = i0; i < i1; i++) {
@@ -472,6 +494,7 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
     run_rows(std::integral_constant<int, 2>{});
   else
     run_rows(std::integral_constant<int, 0>{});
+  if (kGuard && kstep >= nt) return;
   const bool snap = kLast && (kstep % g.Fr) == 0 && kstep / g.Fr < g.nrec_cap;
 
   // Theta edge rows (kern.cu:320-409), then their rho edges.
@@ -521,17 +544,17 @@ __device__ __forceinline__ void stage_tile(const Grid &g, const DevState &d, int
     for (int b = b0; b < b0 + kRows; b++) {
       const double2 q = ld2(out, 16u * (unsigned)(b * N + a));
       const double q0 = q.x, q1 = q.y;
-      const double ir = ext.integ[b] + q0 * dt;
-      const double ii = ext.integ[M + b] + q1 * dt;
-      const double dr = ext.integ[2 * M + b] + ir * dt;
-      const double di = ext.integ[3 * M + b] + ii * dt;
-      ext.integ[b] = ir;
-      ext.integ[M + b] = ii;
-      ext.integ[2 * M + b] = dr;
-      ext.integ[3 * M + b] = di;
+      const double ir = integ[b] + q0 * dt;
+      const double ii = integ[M + b] + q1 * dt;
+      const double dr = integ[2 * M + b] + ir * dt;
+      const double di = integ[3 * M + b] + ii * dt;
+      integ[b] = ir;
+      integ[M + b] = ii;
+      integ[2 * M + b] = dr;
+      integ[3 * M + b] = di;
       if (!isfinite(q0) || !isfinite(q1)) ctl->status = 1;
       if (snap) {
-        double *sn = ext.snap + (size_t)(kstep / g.Fr) * 6 * M;
+        TEUK_GLOBAL double *sn = snap_buf + (size_t)(kstep / g.Fr) * 6 * M;
         sn[b] = q0;
         sn[M + b] = q1;
         sn[2 * M + b] = ir;
@@ -554,21 +577,20 @@ void k_stage(Grid g, DevState d, int has_src, int ngroups, int k_rel) {
   int tile = blockIdx.x;
   if ((nblocks & 7) == 0) tile = (blockIdx.x & 7) * (nblocks >> 3) + (blockIdx.x >> 3);
   const int strip = tile / ngroups, grp = tile - strip * ngroups;
-  stage_tile<INTEG, STAGE>(g, d, has_src, k_rel, strip, grp, blockDim.y);
+  stage_tile<INTEG, STAGE, false>(g, d, has_src, k_rel, strip, grp, blockDim.y, 0);
 }
 
-// Batched stage launch: block i works on tile tiles[i] = {solve, tile} of the
-// batch (the host orders the table for the XCD dealing).  A solve that has
-// completed its scheduled steps is skipped.
+// Batched stage launch: block i works on tiles[i], a member's description
+// and one of its tiles (the host orders the table for the XCD dealing).  A
+// member that has completed its scheduled steps stores nothing.
 template <int INTEG, int STAGE, int W>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W)))
-void k_stage_batch(const SolveDesc *__restrict__ descs, const int2 *__restrict__ tiles, int k_rel) {
-  const int2 t = tiles[blockIdx.x];
-  if (t.x < 0) return;  // padding of the tile table
-  const SolveDesc &sd = descs[t.x];
-  if (sd.d.ctl->k + k_rel >= sd.nt) return;
-  const int strip = t.y / sd.ngroups, grp = t.y - strip * sd.ngroups;
-  stage_tile<INTEG, STAGE>(sd.g, sd.d, sd.has_src, k_rel, strip, grp, blockDim.y);
+void k_stage_batch(const BatchTile *__restrict__ tiles, int k_rel) {
+  const BatchTile &bt = tiles[blockIdx.x];
+  if (bt.tile < 0) return;  // padding of the tile table
+  const SolveDesc &sd = bt.sd;
+  const int strip = bt.tile / sd.ngroups, grp = bt.tile - strip * sd.ngroups;
+  stage_tile<INTEG, STAGE, true>(sd.g, sd.d, sd.has_src, k_rel, strip, grp, blockDim.y, sd.nt);
 }
 
 // ---------------------------------------------------------------------------
@@ -749,12 +771,12 @@ static hipError_t stage_one(const Grid &g, const DevState &d, int hs, int ngroup
 }
 
 template <int I, int S>
-static hipError_t stage_batch_one(const SolveDesc *descs, const int2 *tiles, int ntiles, int k_rel, dim3 block,
-                                  int wpe, hipStream_t s) {
+static hipError_t stage_batch_one(const BatchTile *tiles, int ntiles, int k_rel, dim3 block, int wpe,
+                                  hipStream_t s) {
   if (wpe == 4)
-    hipLaunchKernelGGL((k_stage_batch<I, S, 4>), dim3(ntiles), block, 0, s, descs, tiles, k_rel);
+    hipLaunchKernelGGL((k_stage_batch<I, S, 4>), dim3(ntiles), block, 0, s, tiles, k_rel);
   else if (wpe == 2)
-    hipLaunchKernelGGL((k_stage_batch<I, S, 2>), dim3(ntiles), block, 0, s, descs, tiles, k_rel);
+    hipLaunchKernelGGL((k_stage_batch<I, S, 2>), dim3(ntiles), block, 0, s, tiles, k_rel);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
@@ -784,24 +806,24 @@ hipError_t launch_stage(const Grid &g, const DevState &d, int integrator, int st
   }
 }
 
-hipError_t launch_stage_batch(const SolveDesc *descs, const int2 *tiles, int ntiles, int integrator, int stage,
-                              int k_rel, Tiling t, hipStream_t s) {
+hipError_t launch_stage_batch(const BatchTile *tiles, int ntiles, int integrator, int stage, int k_rel, Tiling t,
+                              hipStream_t s) {
   if (ntiles <= 0) return hipSuccess;
   if (t.wpb < 1 || t.wpb > 4) return hipErrorInvalidValue;
   const dim3 block(kWave, t.wpb);
   if (integrator == 0) {
     switch (stage) {
-      case 1: return stage_batch_one<0, 1>(descs, tiles, ntiles, k_rel, block, t.wpe, s);
-      case 2: return stage_batch_one<0, 2>(descs, tiles, ntiles, k_rel, block, t.wpe, s);
-      case 3: return stage_batch_one<0, 3>(descs, tiles, ntiles, k_rel, block, t.wpe, s);
+      case 1: return stage_batch_one<0, 1>(tiles, ntiles, k_rel, block, t.wpe, s);
+      case 2: return stage_batch_one<0, 2>(tiles, ntiles, k_rel, block, t.wpe, s);
+      case 3: return stage_batch_one<0, 3>(tiles, ntiles, k_rel, block, t.wpe, s);
       default: return hipErrorInvalidValue;
     }
   }
   switch (stage) {
-    case 1: return stage_batch_one<1, 1>(descs, tiles, ntiles, k_rel, block, t.wpe, s);
-    case 2: return stage_batch_one<1, 2>(descs, tiles, ntiles, k_rel, block, t.wpe, s);
-    case 3: return stage_batch_one<1, 3>(descs, tiles, ntiles, k_rel, block, t.wpe, s);
-    case 4: return stage_batch_one<1, 4>(descs, tiles, ntiles, k_rel, block, t.wpe, s);
+    case 1: return stage_batch_one<1, 1>(tiles, ntiles, k_rel, block, t.wpe, s);
+    case 2: return stage_batch_one<1, 2>(tiles, ntiles, k_rel, block, t.wpe, s);
+    case 3: return stage_batch_one<1, 3>(tiles, ntiles, k_rel, block, t.wpe, s);
+    case 4: return stage_batch_one<1, 4>(tiles, ntiles, k_rel, block, t.wpe, s);
     default: return hipErrorInvalidValue;
   }
 }

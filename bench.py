@@ -131,10 +131,12 @@ def batch_section(scale, grid=None):
     log(f"bench: plain loop {t_plain:.2f} s")
     shapes = [(p.N, p.M) for p, _ in items]
     tiling = tune.best_batch_tiling(shapes) or {"waves_per_block": 4, "waves_per_eu": 2, "order": 0, "chunk": 32}
+    groups = tune.best_batch_groups(shapes)
     t0 = time.perf_counter()
     solvers = []
     with teuk.Batch(integrator=1) as b:
         b.set_tiling(**tiling)
+        b.set_cache_groups(groups)
         for p, rows in items:
             s = teuk.Solver(p)
             s.set_trajectory(rows, p.nsteps())
@@ -152,7 +154,8 @@ def batch_section(scale, grid=None):
                         f"({scale:g} of 1200), RK4, plunge source",
             "solves_per_hour": 3600.0 * len(items) / t_batch, "plain_solves_per_hour": 3600.0 * len(items) / t_plain,
             "speedup_vs_plain": t_plain / t_batch, "point_updates_per_s": total / t_batch,
-            "plain_point_updates_per_s": total / t_plain, "tiling": tiling, "members_bitwise_equal_alone": same}
+            "plain_point_updates_per_s": total / t_plain, "tiling": tiling, "cache_group_bytes": groups,
+            "members_bitwise_equal_alone": same}
 
 
 def serve_section(scale, seconds, policy="sjf"):

@@ -9,12 +9,15 @@ Candidates (all give bitwise-identical results; only speed differs):
   order            0, 1     -- batch tile table: contiguous run per XCD, or
                                dealt in list order
   chunk            16, 32, 64 -- steps per captured hipGraph
+  group_bytes      0, 256 MB  -- batch cache groups: off, or members stepped
+                               a chunk at a time in groups whose field state
+                               fits the last-level cache
 
 tune_solver() times single-solve stage launches with HIP events;
 tune_batch() times whole batch steps.  Winners are kept in a JSON cache
 (TEUK_TUNE_CACHE, default ~/.cache/bh_ringdown_amd/tuning.json) keyed by
 grid shape and integrator, and looked up by best_solver_tiling() /
-best_batch_tiling().
+best_batch_tiling() / best_batch_groups().
 """
 import itertools
 import json
@@ -60,7 +63,11 @@ def solver_key(N, M, integrator):
 
 
 def batch_key(shapes, integrator):
-    return "batch/" + ",".join(f"{n}x{m}" for n, m in sorted(set(shapes))) + f"/i{integrator}"
+    """Shapes with their counts: "batch/4096x32*24,8192x32*32,16384x64*16/i1"."""
+    counts = {}
+    for sh in shapes:
+        counts[sh] = counts.get(sh, 0) + 1
+    return "batch/" + ",".join(f"{n}x{m}*{c}" for (n, m), c in sorted(counts.items())) + f"/i{integrator}"
 
 
 def _probe_params(N, M, integrator, steps):
@@ -94,16 +101,23 @@ def tune_solver(N, M, integrator=1, steps=60, repeats=2, store=True):
     return best, times
 
 
-def tune_batch(shapes, integrator=1, steps=64, store=True, chunks=(16, 32, 64)):
-    """Time whole batch steps of probe solvers of the given (N, M) shapes for
-    every (waves_per_block, waves_per_eu, order, chunk); returns
-    (best, {candidate: seconds per step})."""
+# Cache-group sizes tried for batches: off, and groups whose field state fits
+# the 256 MB last-level cache.
+GROUP_BYTES = (0, 256 << 20)
+
+
+def tune_batch(shapes, integrator=1, steps=64, store=True, chunks=(16, 32, 64), group_bytes=GROUP_BYTES):
+    """Time whole batch steps of probe solvers of the given (N, M) shapes
+    (one probe per entry, so repeat a shape to tune a batch of many) for
+    every (waves_per_block, waves_per_eu, order, chunk, cache-group bytes);
+    returns (best, {candidate: seconds per step})."""
     times = {}
-    for wpb, wpe, order, chunk in itertools.product((1, 2, 4), (2, 4), (0, 1), chunks):
+    for wpb, wpe, order, chunk, gb in itertools.product((1, 2, 4), (2, 4), (0, 1), chunks, group_bytes):
         solvers = [teuk.Solver(_probe_params(n, m, integrator, 3 * steps + 10)) for n, m in shapes]
         try:
             with teuk.Batch(integrator=integrator) as b:
                 b.set_tiling(wpb, wpe, order, chunk)
+                b.set_cache_groups(gb)
                 for s in solvers:
                     b.add(s)
                 b.step(chunk)
@@ -111,14 +125,15 @@ def tune_batch(shapes, integrator=1, steps=64, store=True, chunks=(16, 32, 64)):
                 t0 = time.perf_counter()
                 b.step(2 * steps)
                 b.sync()
-                times[(wpb, wpe, order, chunk)] = (time.perf_counter() - t0) / (2 * steps)
+                times[(wpb, wpe, order, chunk, gb)] = (time.perf_counter() - t0) / (2 * steps)
         finally:
             for s in solvers:
                 s.close()
     best = min(times, key=times.get)
     if store:
         _store(batch_key(shapes, integrator), {"waves_per_block": best[0], "waves_per_eu": best[1],
-                                               "order": best[2], "chunk": best[3], "s_per_step": times[best]})
+                                               "order": best[2], "chunk": best[3], "group_bytes": best[4],
+                                               "s_per_step": times[best]})
     return best, times
 
 
@@ -130,22 +145,31 @@ def best_solver_tiling(N, M, integrator=1):
 
 def best_batch_tiling(shapes, integrator=1):
     """Tuned batch schedule as set_tiling keyword arguments, or None.  Falls
-    back to the entry for the same set of row counts M if the exact shape
-    set was not tuned."""
+    back to the entry for the same set of shapes (any counts), then to one
+    for the same set of row counts M, if the exact batch was not tuned."""
     data = _load()
     e = data.get(batch_key(shapes, integrator))
+
+    def kshapes(k):
+        return {tuple(map(int, x.split("*")[0].split("x"))) for x in k.split("/")[1].split(",")}
+
     if e is None:
-        ms = sorted({m for _, m in shapes})
-        for k, v in data.items():
-            if k.startswith("batch/") and k.endswith(f"/i{integrator}"):
-                kms = sorted({int(x.split("x")[1]) for x in k.split("/")[1].split(",")})
-                if kms == ms:
-                    e = v
-                    break
+        cands = [(k, v) for k, v in sorted(data.items()) if k.startswith("batch/") and k.endswith(f"/i{integrator}")]
+        e = next((v for k, v in cands if kshapes(k) == set(shapes)), None)
+        if e is None:
+            ms = {m for _, m in shapes}
+            e = next((v for k, v in cands if {m for _, m in kshapes(k)} == ms), None)
     if not e:
         return None
     return {"waves_per_block": e["waves_per_block"], "waves_per_eu": e["waves_per_eu"], "order": e["order"],
             "chunk": e["chunk"]}
+
+
+def best_batch_groups(shapes, integrator=1):
+    """Tuned cache-group bytes for a batch (0 = off, also when not tuned)."""
+    data = _load()
+    e = data.get(batch_key(shapes, integrator))
+    return int(e.get("group_bytes", 0)) if e else 0
 
 
 def main(argv=None):
@@ -162,7 +186,10 @@ def main(argv=None):
         print(json.dumps({"solver": s, "best": best, "us": {f"{k[0]},{k[1]}": round(v, 3) for k, v in t.items()}}),
               flush=True)
     for spec in args.batch:
-        shapes = [tuple(map(int, x.split("x"))) for x in spec.split(",")]
+        shapes = []
+        for x in spec.split(","):   # NxM or NxM*count
+            sh, _, cnt = x.partition("*")
+            shapes += [tuple(map(int, sh.split("x")))] * int(cnt or 1)
         best, t = tune_batch(shapes, args.integrator)
         print(json.dumps({"batch": spec, "best": best,
                           "ms_per_step": {",".join(map(str, k)): round(1e3 * v, 4) for k, v in t.items()}}),

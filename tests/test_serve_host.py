@@ -224,3 +224,19 @@ def test_server_retires_a_request_whose_setup_fails(monkeypatch):
     assert res[1].failed and srv.stats["failed"] == 1
     assert not res[0].failed and not res[2].failed
     assert not math.isnan(res[0].done) and not math.isnan(res[2].done)
+
+
+def test_batch_schedule_lookup(tmp_path, monkeypatch):
+    """Batch keys count each shape; an exact entry gives the tiling and the
+    cache-group size, the same shapes in other counts fall back to it, and
+    an untuned batch gets no groups."""
+    monkeypatch.setattr(tune, "CACHE", str(tmp_path / "tuning.json"))
+    small = [(4096, 32)] * 72
+    assert tune.batch_key(small, 1) == "batch/4096x32*72/i1"
+    assert tune.batch_key([(8192, 32), (4096, 32), (8192, 32)], 1) == "batch/4096x32*1,8192x32*2/i1"
+    defaults = json.load(open(tune.DEFAULTS))
+    e = defaults[tune.batch_key(small, 1)]
+    assert tune.best_batch_groups(small) == e["group_bytes"]
+    assert tune.best_batch_tiling([(4096, 32)] * 8)["chunk"] == e["chunk"]   # same shapes, other count
+    assert tune.best_batch_groups([(4096, 32)] * 8) == 0                     # groups only for the tuned batch
+    assert tune.best_batch_tiling([(2048, 16)]) is None

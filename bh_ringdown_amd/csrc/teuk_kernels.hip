@@ -601,8 +601,16 @@ void k_stage_batch(const BatchTile *__restrict__ tiles, int k_rel) {
 // round-to-nearest operations in the reference's evaluation order make the
 // records equal the host computation bit for bit.  Then advances the step
 // counter by n.
-__device__ __forceinline__ void finalize_solve(const Grid &g, const Ext &ext, Ctl *ctl, int n) {
+__device__ __forceinline__ void finalize_solve(const Grid &g, const Ext &ext_in, Ctl *ctl_in, int n) {
   const int M = g.M, t = threadIdx.x;
+  // global-memory views of the solve's buffers (see global_ptr)
+  struct {
+    const TEUK_GLOBAL double *snap, *extw, *projw;
+    TEUK_GLOBAL double *rec;
+    TEUK_GLOBAL int *rec_k;
+  } ext = {global_ptr(const_cast<const double *>(ext_in.snap)), global_ptr(ext_in.extw), global_ptr(ext_in.projw),
+           global_ptr(ext_in.rec), global_ptr(ext_in.rec_k)};
+  TEUK_GLOBAL Ctl *ctl = global_ptr(ctl_in);
   const int k0 = ctl->k;
   const double timer0 = ctl->timer;
   const int r_lo = (k0 + g.Fr - 1) / g.Fr;                 // first record in [k0, k0+n)
@@ -613,8 +621,8 @@ __device__ __forceinline__ void finalize_solve(const Grid &g, const Ext &ext, Ct
   const int nterm = 5 * (M / 4);
   for (int task = t; task < nrec * 15; task += blockDim.x) {
     const int r = r_lo + task / 15, part = task % 15;
-    const double *sn = ext.snap + (size_t)r * 6 * M;
-    double *rec = ext.rec + (size_t)r * 16;
+    const TEUK_GLOBAL double *sn = ext.snap + (size_t)r * 6 * M;
+    TEUK_GLOBAL double *rec = ext.rec + (size_t)r * 16;
     // The sums run in the reference's order; their inputs are loaded a group
     // at a time (M is a multiple of 8, so groups of 8 rows and of 5 Boole
     // terms divide the loops) so the loads overlap instead of each waiting
@@ -640,8 +648,8 @@ __device__ __forceinline__ void finalize_solve(const Grid &g, const Ext &ext, Ct
       rec[1] = dedt;
     } else {
       const int l = part >> 1, im = part & 1;
-      const double *wl = ext.projw + (size_t)l * nterm * 3;
-      const double *dq = sn + (4 + im) * M;
+      const TEUK_GLOBAL double *wl = ext.projw + (size_t)l * nterm * 3;
+      const TEUK_GLOBAL double *dq = sn + (4 + im) * M;
       double h = 0.0;
       for (int g5 = 0; g5 < nterm / 5; g5++) {   // Boole group g5: rows 4 g5 .. 4 g5 + 4
         double v[5], w[15];
@@ -699,9 +707,10 @@ __global__ __launch_bounds__(256) void k_finalize_batch(const SolveDesc *__restr
 // the 2 N M pairs of the state.
 __device__ __forceinline__ void sentinel_scan(const double2 *__restrict__ u, size_t n2, Ctl *ctl, size_t i0,
                                               size_t stride) {
+  const TEUK_GLOBAL dvec2 *__restrict__ ug = global_ptr(reinterpret_cast<const dvec2 *>(u));
   bool bad = false;
   for (size_t i = i0; i < n2; i += stride) {
-    const double2 v = u[i];
+    const dvec2 v = ug[i];
     bad = bad || !(fabs(v.x) <= kGrowthLimit) || !(fabs(v.y) <= kGrowthLimit);
   }
   if (bad) atomicOr(&ctl->status, 2);

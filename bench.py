@@ -109,26 +109,28 @@ def cpu_baseline(p, rows, seconds=12.0):
                       f"{p.N}x{p.M} plunge solve; oracle g++ -O3 OpenMP, set-up excluded"}
 
 
-def batch_section(scale, grid=None):
+def batch_section(scale, grid=None, plain=True):
     """The 72 uneven solves (9 m x 8 spins, grids by spin, or all on `grid`)
     at `scale` of their final time: the plain one-solve-at-a-time loop
-    against one batch."""
+    against one batch (plain=False: the batch alone, for profiler runs of
+    the batched kernels)."""
     from bh_ringdown_amd import teuk, tune, workloads
     items = workloads.batch_72(seed=0, t_final=1200.0 * scale, grid=grid)
     total = sum(p.N * p.M * p.nsteps() for p, _ in items)
     log(f"bench: batch of {len(items)} solves, {total:.3e} point-updates")
-    probe = {0, 31, 71}
+    probe = {0, 31, 71} if plain else set()
     t0 = time.perf_counter()
     plain_out = {}
-    for i, (p, rows) in enumerate(items):
+    for i, (p, rows) in enumerate(items if plain else []):
         with teuk.Solver(p) as s:
             s.set_trajectory(rows, p.nsteps())
             s.step(p.nsteps())
             s.sync()
             if i in probe:
                 plain_out[i] = s.output()["h"]
-    t_plain = time.perf_counter() - t0
-    log(f"bench: plain loop {t_plain:.2f} s")
+    t_plain = time.perf_counter() - t0 if plain else None
+    if plain:
+        log(f"bench: plain loop {t_plain:.2f} s")
     shapes = [(p.N, p.M) for p, _ in items]
     tiling = tune.best_batch_tiling(shapes) or {"waves_per_block": 4, "waves_per_eu": 2, "order": 0, "chunk": 32}
     groups = tune.best_batch_groups(shapes)
@@ -152,10 +154,11 @@ def batch_section(scale, grid=None):
     grids = f"all {grid[0]}x{grid[1]}" if grid else "N x M by spin in 4096x32, 8192x32, 16384x64"
     return {"workload": f"72 solves (m -4..4 x 8 seeded spins; {grids}) to t_final {1200.0 * scale:g} "
                         f"({scale:g} of 1200), RK4, plunge source",
-            "solves_per_hour": 3600.0 * len(items) / t_batch, "plain_solves_per_hour": 3600.0 * len(items) / t_plain,
-            "speedup_vs_plain": t_plain / t_batch, "point_updates_per_s": total / t_batch,
-            "plain_point_updates_per_s": total / t_plain, "tiling": tiling, "cache_group_bytes": groups,
-            "members_bitwise_equal_alone": same}
+            "solves_per_hour": 3600.0 * len(items) / t_batch,
+            "plain_solves_per_hour": 3600.0 * len(items) / t_plain if plain else None,
+            "speedup_vs_plain": t_plain / t_batch if plain else None, "point_updates_per_s": total / t_batch,
+            "plain_point_updates_per_s": total / t_plain if plain else None, "tiling": tiling,
+            "cache_group_bytes": groups, "members_bitwise_equal_alone": same if plain else None}
 
 
 def serve_section(scale, seconds, policy="sjf"):
@@ -199,6 +202,10 @@ def main():
     ap.add_argument("--wpe", type=int, default=0, help="occupancy target, waves per SIMD (0 = default)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--batch-sections", choices=["both", "mixed", "small"], default="both",
+                    help="which batch sections run: the mixed 72 solves, the 72 at 4096x32, or both")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="batch sections without the one-at-a-time comparison (profiler runs of the batched kernels)")
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the batch/serve solve lengths")
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--serve-policy", choices=["fifo", "sjf"], default="sjf",
@@ -289,8 +296,10 @@ def main():
                      "frac_counter": traffic / per_launch_s / 1e9 / HBM_PEAK_GBS if traffic else None},
     }
     if not args.no_batch:
-        out["batch"] = batch_section(args.scale)
-        out["batch_small_grids"] = batch_section(args.scale, grid=(4096, 32))
+        if args.batch_sections in ("both", "mixed"):
+            out["batch"] = batch_section(args.scale, plain=not args.no_plain)
+        if args.batch_sections in ("both", "small"):
+            out["batch_small_grids"] = batch_section(args.scale, grid=(4096, 32), plain=not args.no_plain)
     if not args.no_serve:
         out["serving"] = serve_section(args.scale, args.serve_seconds, args.serve_policy)
     if not args.no_cpu:

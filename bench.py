@@ -281,8 +281,8 @@ def main():
                    "t_final": p.T,
                    "source": "seeded plunge, mass ratio 1e-3, inclination 30 deg, from 1.02 r_isco",
                    "extraction_rho": p.xmin + p.j_extract * p.dx},
-        # headline: algorithmic bytes (the survey's model: RK4 512 B, SSP-RK3
-        # 256 B per point-update) over the average stage launch; beside it the
+        # headline: algorithmic bytes (RK4 in its accumulator form 512 B,
+        # SSP-RK3 256 B per point-update) over the average stage launch; beside it the
         # state bytes the kernels move (RK4 stage-value form 384 B) and the
         # counter-measured HBM bytes per launch (committed rocprofv3 --pmc
         # summary of this grid), each over the same launch time

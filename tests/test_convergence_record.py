@@ -4,7 +4,7 @@ from the committed record tests/golden/convergence_a099.json
 GPU, grids doubling, psi at rho = 1000 compared at shared times and theta
 rows).  Orders are log2(|psi_1 - psi_2| / |psi_2 - psi_4|) over 250 M
 windows; only windows where the wave is present (rms psi above 1e-2, or
-1e-3 for the plunge) are judged -- before the pulse arrives the differences
+1e-3 for the plunge) are assessed -- before the pulse arrives the differences
 are zero and after it has passed they are a 1e-6 late-time residue.
 
 * rho and dt refined, theta fixed, Gaussian data: 4th order (measured 4.3
@@ -25,7 +25,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REC = json.load(open(os.path.join(HERE, "golden", "convergence_a099.json")))
 
 
-def judged(study, floor):
+def signal_windows(study, floor):
     return [w for w in REC[study]["windows"] if w["rms_psi"] > floor]
 
 
@@ -37,19 +37,19 @@ def test_record_grids():
 
 
 def test_rho_refinement_is_fourth_order():
-    ws = judged("rho", 1e-2)
+    ws = signal_windows("rho", 1e-2)
     assert len(ws) >= 2
     assert all(w["order"] > 3.8 for w in ws), [w["order"] for w in ws]
 
 
 def test_theta_refinement_is_about_second_order():
-    ws = judged("theta", 1e-2)
+    ws = signal_windows("theta", 1e-2)
     assert len(ws) >= 2
     assert all(1.4 < w["order"] < 2.3 for w in ws), [w["order"] for w in ws]
 
 
 def test_plunge_converges():
-    ws = judged("plunge", 1e-3)
+    ws = signal_windows("plunge", 1e-3)
     assert len(ws) >= 2
     assert all(w["diff_24"] < w["diff_12"] and w["order"] > 0.3 for w in ws), [w["order"] for w in ws]
     # where the signal is largest the finest pair still differs by 81 % of

@@ -170,3 +170,25 @@ def test_driver_rejects_unknown_parameter(tmp_path):
     res = subprocess.run([driver, "-p", str(params)], capture_output=True, text=True, timeout=60)
     assert res.returncode == 2
     assert "unknown parameter 'Ntheta'" in res.stderr
+
+
+def test_body_binding_compiles_against_the_parameter_macros(tmp_path):
+    """driver/body_teuk.cpp -- body() over libteuk for a build that keeps the
+    reference's main.c -- compiles with the parameter file's macros (N, M, T,
+    Fr, ... as in parm.h:1-13) in scope and defines body() with the C++
+    signature main.c:21-23 declares (main.c is compiled as C++, build.sh:7):
+    body(double*, double*, double x 7, int, int)."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    obj = tmp_path / "body_teuk.o"
+    r = subprocess.run([gxx, "-std=c++17", "-O1", "-Wall", "-Werror", "-c",
+                        "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "oracle", "body_parm"),
+                        os.path.join(ROOT, "bh_ringdown_amd", "driver", "body_teuk.cpp"), "-o", str(obj)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    syms = subprocess.run(["nm", str(obj)], capture_output=True, text=True).stdout
+    assert re.search(r"\bT _Z4bodyPdS_dddddddii\b", syms), syms
+    assert re.search(r"\bU teuk_body\b", syms) and re.search(r"\bU teuk_strerror\b", syms)

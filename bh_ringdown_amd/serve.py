@@ -222,6 +222,13 @@ class Server:
                     # runs; admit when it is ready (wait for it only with
                     # nothing resident to step).
                     if req.id not in prepared:
+                        # keep the host set-ups of the next few waiting
+                        # requests only (shortest-first admission can move
+                        # a prepared request back in the queue)
+                        if len(prepared) >= 8:
+                            near = {r.id for r in queue[:8]}
+                            for rid in [rid for rid in prepared if rid not in near]:
+                                del prepared[rid]
                         prepared[req.id] = worker.submit(timed_prepare, req)
                     if live and not prepared[req.id].done():
                         break

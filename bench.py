@@ -71,14 +71,26 @@ def pmc_traffic(integrator, N, M):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(p, rows, seconds=12.0):
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(p, rows, seconds=12.0, threads=None):
     """The oracle (g++ -O3 OpenMP build) on the first steps of the same
     workload: source, the RK stages, boundaries, extraction and projection
     per step.  The one-off coefficient set-up is timed separately and
     subtracted (a one-step run)."""
     from bh_ringdown_amd import teuk
     from oracle import oracle as O
-    nth = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    nth = threads or int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
     _, r_c, th = teuk.grid(p)
     op = O.make_params(p.N, p.M, p.xmin, p.xmax, p.a, p.m, dt=p.dt, j_extract=p.j_extract, nthreads=nth,
                        integrator=p.integrator)
@@ -106,7 +118,8 @@ def cpu_baseline(p, rows, seconds=12.0):
     el = timed(n + 1) - t1
     return {"value": p.N * p.M * n / el, "unit": "point-updates/s", "cores": nth, "kind": "port",
             "sample": f"first {n} {name} steps (with source, extraction, projection) of the "
-                      f"{p.N}x{p.M} plunge solve; oracle g++ -O3 OpenMP, set-up excluded"}
+                      f"{p.N}x{p.M} plunge solve; oracle g++ -O3 OpenMP, set-up excluded",
+            "cpu": cpu_model()}
 
 
 def batch_section(scale, grid=None, plain=True):
@@ -305,6 +318,9 @@ def main():
     if not args.no_cpu:
         log("bench: cpu baseline")
         out["cpu_baseline"] = cpu_baseline(p, rows)
+        one = cpu_baseline(p, rows, seconds=6.0, threads=1)
+        out["cpu_baseline"]["value_1_core"] = one["value"]
+        out["cpu_baseline"]["sample_1_core"] = one["sample"]
     print(json.dumps(out), flush=True)
     return 0
 

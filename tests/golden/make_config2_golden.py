@@ -1,9 +1,9 @@
-"""Writes tests/golden/config2_rk4_records.npz: the oracle's records of the
-full single solve -- N = 16384, M = 64, rho in [-400, 1200], a = 0.9, m = 2,
-classic RK4, dt = 0.5 drho, zero initial fields, the seeded plunge (mass
-ratio 1e-3, inclination 30 degrees, from 1.02 r_isco), psi at rho = 1000,
-t = 1500 (30720 steps), output every 2 steps -- for the GPU test
-tests/test_gpu_config2.py:
+"""Writes tests/golden/config2_<integrator>_records.npz: the oracle's records
+of the full single solve -- N = 16384, M = 64, rho in [-400, 1200], a = 0.9,
+m = 2, classic RK4 (or the reference's SSP-RK3), dt = 0.5 drho, zero initial
+fields, the seeded plunge (mass ratio 1e-3, inclination 30 degrees, from
+1.02 r_isco), psi at rho = 1000, t = 1500 (30720 steps), output every 2
+steps -- for the GPU test tests/test_gpu_configs.py:
 
   * every 10th output record (h_lm for l = 2..8, dE/dt) and its step;
   * psi at the extraction column every 100th record;
@@ -11,9 +11,10 @@ tests/test_gpu_config2.py:
   * the QNM amplitudes and phases fitted to the full h record (the fit
     anchored at the peak of |h_22|), with the a = 0.9, m = 2 table rows.
 
-The oracle (test infrastructure) takes about half an hour on 8 host threads:
+The oracle (test infrastructure) takes one to one and a half hours on six
+host threads:
 
-    python tests/golden/make_config2_golden.py [--threads 6]
+    python tests/golden/make_config2_golden.py [--threads 6] [--integrator rk4|rk3]
 """
 import argparse
 import hashlib
@@ -34,8 +35,11 @@ from oracle import oracle as O  # noqa: E402
 H_EVERY, PSI_EVERY = 10, 100
 
 
-def config2():
-    p = workloads.single_16384x64(integrator=1)
+INTEGRATORS = {"rk3": 0, "rk4": 1}
+
+
+def config2(integrator="rk4"):
+    p = workloads.single_16384x64(integrator=INTEGRATORS[integrator])
     nsteps = p.nsteps()
     rows = workloads.plunge_rows(p, nsteps)
     return p, nsteps, rows
@@ -64,15 +68,17 @@ def traj_digest(rows):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--threads", type=int, default=O.DEFAULT_THREADS)
+    ap.add_argument("--integrator", choices=sorted(INTEGRATORS), default="rk4")
     args = ap.parse_args()
-    p, nsteps, rows = config2()
+    p, nsteps, rows = config2(args.integrator)
+    name = f"config2_{args.integrator}_records.npz"
     _, r_c, th = teuk.grid(p)
     sched = teuk.trajectory_schedule(rows, nsteps, p.dt)
     ost = [O.SrcStep(*[getattr(x, f) for f, _ in teuk.SourceStep._fields_]) for x in sched]
     op = O.make_params(p.N, p.M, p.xmin, p.xmax, p.a, p.m, dt=p.dt, integrator=p.integrator,
                        j_extract=p.j_extract, nthreads=args.threads)
     u = np.zeros((4, p.M, p.N))
-    print(f"oracle: {p.N}x{p.M} RK4 plunge, {nsteps} steps on {args.threads} threads", flush=True)
+    print(f"oracle: {p.N}x{p.M} {args.integrator} plunge, {nsteps} steps on {args.threads} threads", flush=True)
     t0 = time.time()
     rc, psi, h, dedt = O.run(op, r_c, th, u, nsteps, src_steps=ost)
     print(f"oracle: rc {rc}, {time.time() - t0:.0f} s", flush=True)
@@ -82,13 +88,13 @@ def main():
     amp, phase = fit_qnm(t, h[ks], p.a, int(p.m), tables_a09())
     sel_h = ks[::H_EVERY]
     sel_p = ks[::PSI_EVERY]
-    np.savez_compressed(os.path.join(HERE, "config2_rk4_records.npz"),
+    np.savez_compressed(os.path.join(HERE, name),
                         nsteps=nsteps, Fr=p.Fr, h_k=sel_h, h=h[sel_h], dedt=dedt[sel_h], psi_k=sel_p, psi=psi[sel_p],
                         fields_sha256=np.frombuffer(hashlib.sha256(np.ascontiguousarray(u).tobytes()).digest(),
                                                     dtype=np.uint8),
                         qnm_amp=amp, qnm_phase=phase, max_abs_field=np.abs(u).max(),
                         traj_sha256=traj_digest(rows))
-    print("wrote config2_rk4_records.npz; l2m2 amplitude", amp[0], "phase", phase[0], flush=True)
+    print(f"wrote {name}; l2m2 amplitude", amp[0], "phase", phase[0], flush=True)
 
 
 if __name__ == "__main__":

@@ -4,8 +4,8 @@ against the oracle (not against another run of the HIP path):
 * config 2 -- N = 16384, M = 64, rho in [-400, 1200], a = 0.9, m = 2, zero
   fields and the seeded plunge (mass ratio 1e-3, inclination 30 degrees):
   the first 256 steps live against the oracle (RK4 and SSP-RK3), and the
-  whole run (t = 1500, 30720 RK4 steps) against the oracle's records
-  (tests/golden/config2_rk4_records.npz, tests/golden/make_config2_golden.py);
+  whole run (t = 1500, 30720 steps, RK4 and SSP-RK3) against the oracle's records
+  (tests/golden/config2_{rk4,rk3}_records.npz, tests/golden/make_config2_golden.py);
 * config 3 -- the 72 batched solves (m = -4..4 x 8 seeded spins, grids
   4096x32 / 8192x32 / 16384x64 by spin, one inclined plunge per spin),
   40 steps each in one batch, every member against the oracle;
@@ -74,12 +74,13 @@ def test_config2_first_256_steps(integrator):
     assert same_as_oracle(ug, out, psig, uo, psio, ho, dedto)
 
 
-def test_config2_full_length_against_oracle_records():
-    path = os.path.join(HERE, "golden", "config2_rk4_records.npz")
+@pytest.mark.parametrize("name,integrator", [("rk4", 1), ("rk3", 0)])
+def test_config2_full_length_against_oracle_records(name, integrator):
+    path = os.path.join(HERE, "golden", f"config2_{name}_records.npz")
     if not os.path.exists(path):
-        pytest.skip("oracle records not generated (tests/golden/make_config2_golden.py)")
+        pytest.skip(f"oracle records not generated (tests/golden/make_config2_golden.py --integrator {name})")
     gold = np.load(path)
-    p = workloads.single_16384x64(integrator=1)
+    p = workloads.single_16384x64(integrator=integrator)
     n = p.nsteps()
     assert n == int(gold["nsteps"]) == 30720
     rows = workloads.plunge_rows(p, n)
@@ -89,7 +90,7 @@ def test_config2_full_length_against_oracle_records():
     if "traj_sha256" not in gold.files or not np.array_equal(G.traj_digest(rows), gold["traj_sha256"]):
         pytest.skip("the records were made from another trajectory (the generator changed since); "
                     "regenerate them with tests/golden/make_config2_golden.py")
-    print(f"config 2 full length: {n} RK4 steps", flush=True)
+    print(f"config 2 full length: {n} {name} steps", flush=True)
     ug, out, psig = gpu_solve(p, rows, n)
     ks = out["k"]
     assert np.array_equal(ks, np.arange(0, n, p.Fr))

@@ -213,6 +213,7 @@ def main():
                          "ladder grids are larger single solves of the same path)")
     ap.add_argument("--wpb", type=int, default=0, help="wavefronts per block (0 = default)")
     ap.add_argument("--wpe", type=int, default=0, help="occupancy target, waves per SIMD (0 = default)")
+    ap.add_argument("--rows", type=int, default=0, help="theta rows per wavefront, 2, 4 or 8 (0 = default)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--batch-sections", choices=["both", "mixed", "small"], default="both",
@@ -232,7 +233,7 @@ def main():
               "(independent solves shard one per GPU with no communication)", file=sys.stderr)
         return 2
 
-    from bh_ringdown_amd import teuk, workloads
+    from bh_ringdown_amd import teuk, tune, workloads
     if args.copy_calibration:
         import torch
         x = torch.ones(256 << 20, dtype=torch.float64, device="cuda")
@@ -252,9 +253,14 @@ def main():
     rows = workloads.plunge_rows(p, nt, eta=1e-3, iota_deg=30.0, seed=0)
     pts = p.N * p.M
     log(f"bench: workload {args.workload} N={p.N} M={p.M} warmup={args.warmup} steps={args.steps}")
+    # launch shape: the tuned one for this grid and integrator (committed
+    # table), unless given on the command line; None = the library default
+    tiling = tune.best_solver_tiling(p.N, p.M, integ)
+    if args.wpb or args.wpe or args.rows:
+        tiling = {"waves_per_block": args.wpb or 1, "waves_per_eu": args.wpe or 2, "theta_rows": args.rows or 2}
     with teuk.Solver(p, r_c, th) as s:
-        if args.wpb or args.wpe:
-            s.set_tiling(args.wpb or 4, args.wpe or 2)
+        if tiling:
+            s.set_tiling(**tiling)
         s.set_trajectory(rows, nt)
         s.step(args.warmup)
         s.sync()
@@ -293,7 +299,8 @@ def main():
                    "integrator": INTEGRATOR_NAMES[args.integrator],
                    "t_final": p.T,
                    "source": "seeded plunge, mass ratio 1e-3, inclination 30 deg, from 1.02 r_isco",
-                   "extraction_rho": p.xmin + p.j_extract * p.dx},
+                   "extraction_rho": p.xmin + p.j_extract * p.dx,
+                   "launch": tiling or "library default"},
         # headline: algorithmic bytes (RK4 in its accumulator form 512 B,
         # SSP-RK3 256 B per point-update) over the average stage launch; beside it the
         # state bytes the kernels move (RK4 stage-value form 384 B) and the

@@ -22,7 +22,10 @@ HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contrac
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wno-unused-result", f"-I{INCLUDE}", f"-I{CSRC}"]
 
-LIB_SOURCES = ["teuk_kernels.hip", "teuk_so2.hip", "teuk_api.hip", "teuk_host.cpp", "teuk_traj.cpp"]
+# The stage kernels are built once per theta-chunk length, one object each
+# (the longest compiles first).
+LIB_SOURCES = ["teuk_stage_r8.hip", "teuk_stage_r4.hip", "teuk_stage_r2.hip", "teuk_kernels.hip", "teuk_so2.hip",
+               "teuk_api.hip", "teuk_host.cpp", "teuk_traj.cpp"]
 
 
 def _run(cmd):

@@ -48,7 +48,7 @@ struct teuk_solver {
 // stage for every member), with graph-captured chunks of steps.
 struct teuk_batch {
   int integrator = 0;
-  Tiling tiling{4, 2};
+  Tiling tiling{4, kDefaultWavesPerEU, kDefaultThetaRows};
   int chunk = 32;          // steps per captured graph
   int order = 0;           // tile table: 0 = contiguous run per XCD, 1 = dealt in list order
   long long budget = 0;    // bytes of member footprint allowed (0 = no limit)
@@ -100,7 +100,7 @@ int dalloc(teuk_solver *s, T **p, size_t count) {
 int validate(const teuk_params *p) {
   if (!p) return TEUK_EINVAL;
   if (p->N < 128 || p->N % kWave != 0) return TEUK_EINVAL;
-  if (p->M < 16 || p->M % kRows != 0) return TEUK_EINVAL;
+  if (p->M < 16 || p->M % 8 != 0) return TEUK_EINVAL;  // the stage kernels' theta chunks (theta_chunks)
   if (p->M > (p->integrator == 2 ? 256 : 1024)) return TEUK_EINVAL;
   if ((long long)p->N * p->M > (1LL << 26)) return TEUK_EINVAL;  // kernels address planes with 32-bit offsets
   if (p->s != -2.0) return TEUK_EINVAL;  // kern.cu:142-145 bake in s = -2
@@ -573,7 +573,7 @@ int create_solver(const teuk_params *p, const double *r_h, const double *r_c, co
   teuk_solver *s = new teuk_solver();
   s->P = *p;
   s->nt = scheduled_steps(p);
-  s->tiling = default_tiling(p->M);
+  s->tiling = default_tiling(p->N, p->M);
   s->g = make_grid(p, s->nt / p->Fr + 1);
   Grid &g = s->g;
   auto fail = [&](int code) {
@@ -961,9 +961,21 @@ int teuk_body(const teuk_params *p, const double *r_c, const double *theta, cons
 int teuk_set_tiling(teuk_solver *s, int waves_per_block, int waves_per_eu) {
   if (!s || s->batch || s->g.stag) return TEUK_EINVAL;
   if (!(waves_per_block == 1 || waves_per_block == 2 || waves_per_block == 4)) return TEUK_EINVAL;
-  if (!(waves_per_eu == 2 || waves_per_eu == 4)) return TEUK_EINVAL;
+  if (!(waves_per_eu >= 2 && waves_per_eu <= 4)) return TEUK_EINVAL;
   s->tiling.wpb = waves_per_block;
   s->tiling.wpe = waves_per_eu;
+  if (s->gexec) {
+    HIPCHK(hipStreamSynchronize(s->stream));
+    (void)hipGraphExecDestroy(s->gexec);
+    s->gexec = nullptr;
+  }
+  return TEUK_OK;
+}
+
+int teuk_set_theta_rows(teuk_solver *s, int rows) {
+  if (!s || s->batch || s->g.stag) return TEUK_EINVAL;
+  if (!theta_rows_ok(s->P.M, rows)) return TEUK_EINVAL;
+  s->tiling.rows = rows;
   if (s->gexec) {
     HIPCHK(hipStreamSynchronize(s->stream));
     (void)hipGraphExecDestroy(s->gexec);
@@ -1009,7 +1021,7 @@ int batch_rebuild(teuk_batch *b) {
     sd.g = s->g;
     sd.d = s->d;
     sd.has_src = s->has_src ? 1 : 0;
-    sd.ngroups = stage_groups(s->P.M, b->tiling.wpb);
+    sd.ngroups = stage_groups(s->P.M, b->tiling.wpb, b->tiling.rows);
     sd.nt = member_steps(s);
     sd.tiles = (s->P.N / kWave) * sd.ngroups;
     if (s->k_host >= sd.nt) continue;
@@ -1125,7 +1137,7 @@ int teuk_batch_create(int integrator, long long hbm_budget, teuk_batch **out) {
   teuk_batch *b = new teuk_batch();
   b->integrator = integrator;
   b->budget = hbm_budget;
-  b->tiling = default_tiling(32);
+  b->tiling = Tiling{1, kDefaultWavesPerEU, kDefaultThetaRows};
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
     delete b;
     return TEUK_EHIP;
@@ -1148,12 +1160,19 @@ void teuk_batch_destroy(teuk_batch *b) {
 int teuk_batch_set_tiling(teuk_batch *b, int waves_per_block, int waves_per_eu, int order, int chunk) {
   if (!b) return TEUK_EINVAL;
   if (!(waves_per_block == 1 || waves_per_block == 2 || waves_per_block == 4)) return TEUK_EINVAL;
-  if (!(waves_per_eu == 2 || waves_per_eu == 4) || (order != 0 && order != 1)) return TEUK_EINVAL;
+  if (!(waves_per_eu >= 2 && waves_per_eu <= 4) || (order != 0 && order != 1)) return TEUK_EINVAL;
   if (chunk < 1 || chunk > 256) return TEUK_EINVAL;
   b->tiling.wpb = waves_per_block;
   b->tiling.wpe = waves_per_eu;
   b->order = order;
   b->chunk = chunk;
+  b->dirty = true;
+  return TEUK_OK;
+}
+
+int teuk_batch_set_theta_rows(teuk_batch *b, int rows) {
+  if (!b || !(rows == 2 || rows == 4 || rows == 8)) return TEUK_EINVAL;
+  b->tiling.rows = rows;
   b->dirty = true;
   return TEUK_OK;
 }
@@ -1392,7 +1411,7 @@ int teuk_solve_noalloc(const teuk_params *p, const double *r_c, const double *th
   // the steps, in the launch sequence of teuk_step: chunks of kRing steps
   // (patches for the chunk, the fused stages of each step), each closed by
   // the sentinel and the output records
-  const Tiling tiling = default_tiling(p->M);
+  const Tiling tiling = default_tiling(p->N, p->M);
   const int nstage = stages_per_step(p->integrator);
   for (int done = 0; done < nsteps;) {
     const int n = std::min(kRing, nsteps - done);

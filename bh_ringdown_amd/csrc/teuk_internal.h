@@ -22,10 +22,9 @@ constexpr double kThird = 0.33333333333333333;
 constexpr double kD1 = 0.016666666666667;
 
 // Stage-kernel tile: one wavefront owns 64 consecutive rho columns and a
-// chunk of kRows consecutive theta rows; up to 4 wavefronts per block stack
-// along theta.
+// chunk of consecutive theta rows (theta_chunks); up to 4 wavefronts per
+// block stack along theta.
 constexpr int kWave = 64;
-constexpr int kRows = 8;
 constexpr int kHalo = 4;            // KO reach in rho (kern.cu:247-256)
 // Source patches are prepared for up to kRing steps per launch and kept in a
 // ring of that many slots (step k uses slot k % kRing).
@@ -152,11 +151,30 @@ struct BatchTile {
   int pad[3];
 };
 
-// Stage-kernel launch shape: wavefronts per block (1, 2 or 4) and the
-// occupancy hint the kernel was compiled for (waves per SIMD: 2 or 4).
+// Theta chunks of one rho strip for wavefronts of R rows (R = 2, 4 or 8):
+// the first chunk computes rows 3 .. 2+E and fills rows 0-2 from rows 3 and
+// 4, the last computes rows M-3-E .. M-4 and fills M-3 .. M-1 from M-5 and
+// M-4, and the rows between go R at a time.  E = edge_rows(R) >= 2 makes R
+// divide what is left for every M that is a multiple of 8 and at least 16
+// (the grids teuk_create accepts).
+TEUK_HD constexpr int edge_rows(int R) { return R == 8 ? 5 : (R == 4 ? 3 : 2); }
+TEUK_HD constexpr int theta_chunks(int M, int R) { return 2 + (M - 2 * edge_rows(R) - 6) / R; }
+TEUK_HD constexpr bool theta_rows_ok(int M, int R) {
+  return (R == 2 || R == 4 || R == 8) && M >= 16 && M % 8 == 0;
+}
+static_assert(theta_chunks(16, 8) == 2 && theta_chunks(64, 8) == 8 && theta_chunks(64, 4) == 15 &&
+                  theta_chunks(64, 2) == 29 && theta_chunks(16, 2) == 5,
+              "theta chunk layout");
+constexpr int kDefaultThetaRows = 2;
+constexpr int kDefaultWavesPerEU = 2;   // a batch's until set (teuk_batch_set_tiling)
+
+// Stage-kernel launch shape: wavefronts per block (1, 2 or 4), the
+// occupancy hint the kernel was compiled for (waves per SIMD: 2 or 4) and
+// the theta rows per wavefront (2, 4 or 8).
 struct Tiling {
   int wpb;
   int wpe;
+  int rows;
 };
 
 // Launchers (teuk_kernels.hip).
@@ -182,14 +200,22 @@ hipError_t launch_source_batch(const SolveDesc *descs, int nsolves, int max_cell
                                hipStream_t s);
 hipError_t launch_stage_batch(const BatchTile *tiles, int ntiles, int integrator, int stage, int k_rel, Tiling t,
                               hipStream_t s);
+// The stage builds for R theta rows per wavefront (teuk_stage_r<R>.hip),
+// behind launch_stage / launch_stage_batch.
+template <int R>
+hipError_t launch_stage_rows(const Grid &g, const DevState &d, int integrator, int stage, bool has_src, int k_rel,
+                             int ngroups, dim3 grid, dim3 block, int wpe, hipStream_t s);
+template <int R>
+hipError_t launch_stage_batch_rows(const BatchTile *tiles, int ntiles, int integrator, int stage, int k_rel,
+                                   dim3 block, int wpe, hipStream_t s);
 hipError_t launch_finalize_batch(const SolveDesc *descs, int nsolves, int n, hipStream_t s);
 // The sentinel of every member still running before the chunk's finalize.
 hipError_t launch_sentinel_batch(const SolveDesc *descs, int nsolves, hipStream_t s);
-// Default launch shape for an M-row grid.
-Tiling default_tiling(int M);
-// Blocks per rho strip for wpb waves per block (the last may hold surplus
-// waves, which return at once).
-int stage_groups(int M, int wpb);
+// Default launch shape for an N x M grid.
+Tiling default_tiling(int N, int M);
+// Blocks per rho strip for wpb waves per block of `rows` theta rows each
+// (the last may hold surplus waves, which return at once).
+int stage_groups(int M, int wpb, int rows);
 bool source_available();
 
 // Host helpers (teuk_host.cpp).

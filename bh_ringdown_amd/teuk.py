@@ -77,7 +77,7 @@ EXPORTS = [
     "teuk_batch_remove", "teuk_batch_set_tiling", "teuk_batch_step", "teuk_batch_sync", "teuk_batch_info",
     "teuk_source_coeffs", "teuk_source_patch", "teuk_grid_half", "teuk_create_staggered",
     "teuk_trajectory_schedule_staggered", "teuk_status", "teuk_solve_noalloc", "teuk_batch_set_cache_groups",
-    "teuk_ot_transition",
+    "teuk_ot_transition", "teuk_set_theta_rows", "teuk_batch_set_theta_rows",
 ]
 
 _lib = None
@@ -130,6 +130,8 @@ def lib():
         "teuk_traj_plunge": ([ctypes.c_double] * 6 + [ctypes.c_ulonglong, _dp, ctypes.c_int,
                                                       ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "teuk_set_tiling": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "teuk_set_theta_rows": ([vp, ctypes.c_int], ctypes.c_int),
+        "teuk_batch_set_theta_rows": ([vp, ctypes.c_int], ctypes.c_int),
         "teuk_ot_transition": ([ctypes.c_double, _dp, _dp], ctypes.c_int),
         "teuk_remaining": ([vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "teuk_source_coeffs": ([vp, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
@@ -341,8 +343,11 @@ class Solver:
         _check(lib().teuk_read_psi(self._h, first, cap, _ptr(buf), ctypes.byref(n)), "teuk_read_psi")
         return buf[: n.value].copy()
 
-    def set_tiling(self, waves_per_block, waves_per_eu):
+    def set_tiling(self, waves_per_block, waves_per_eu, theta_rows=None):
+        """Stage launch shape; theta_rows (2, 4, 8) per wavefront if given."""
         _check(lib().teuk_set_tiling(self._h, int(waves_per_block), int(waves_per_eu)), "teuk_set_tiling")
+        if theta_rows is not None:
+            _check(lib().teuk_set_theta_rows(self._h, int(theta_rows)), "teuk_set_theta_rows")
 
     def source_coeffs(self, first=0, n=None):
         """Device-computed per-step source coefficients, shape (n, 12)."""
@@ -414,9 +419,11 @@ class Batch:
         _check(lib().teuk_batch_remove(self._h, solver._h), "teuk_batch_remove")
         self.members.remove(solver)
 
-    def set_tiling(self, waves_per_block=4, waves_per_eu=2, order=0, chunk=32):
+    def set_tiling(self, waves_per_block=4, waves_per_eu=2, order=0, chunk=32, theta_rows=None):
         _check(lib().teuk_batch_set_tiling(self._h, int(waves_per_block), int(waves_per_eu), int(order),
                                            int(chunk)), "teuk_batch_set_tiling")
+        if theta_rows is not None:
+            _check(lib().teuk_batch_set_theta_rows(self._h, int(theta_rows)), "teuk_batch_set_theta_rows")
 
     def set_cache_groups(self, group_bytes):
         """Step members in groups whose field state fits group_bytes (0 = one group)."""

@@ -1,11 +1,16 @@
 """Autotuner for the stage-kernel launch shape and the batch schedule.
 
 Candidates (all give bitwise-identical results; only speed differs):
-  waves_per_block  1, 2, 4  -- wavefronts (64 rho columns x 8 theta rows each)
-                               stacked along theta in one block
-  waves_per_eu     2, 4     -- occupancy the kernel is compiled for (4 forces
-                               register spilling; kept as a candidate because
-                               the trade-off is measured, not assumed)
+  waves_per_block  1, 2, 4  -- wavefronts (64 rho columns x a chunk of theta
+                               rows each) stacked along theta in one block
+  waves_per_eu     2, 3, 4  -- occupancy the kernel is compiled for (3 fits
+                               2-row wavefronts with a few spilled registers;
+                               4 forces heavy spilling, kept as a candidate
+                               for single solves because the trade-off is
+                               measured, not assumed)
+  theta_rows       2, 4, 8  -- theta rows per wavefront (short chunks: more
+                               wavefronts and fewer registers; long ones:
+                               fewer theta halo re-reads)
   order            0, 1     -- batch tile table: contiguous run per XCD, or
                                dealt in list order
   chunk            16, 32, 64 -- steps per captured hipGraph
@@ -78,14 +83,14 @@ def _probe_params(N, M, integrator, steps):
     return p
 
 
-def tune_solver(N, M, integrator=1, steps=60, repeats=2, store=True):
-    """Time every (waves_per_block, waves_per_eu) on an N x M grid; returns
-    (best, {candidate: us per stage launch})."""
+def tune_solver(N, M, integrator=1, steps=60, repeats=2, store=True, rows=(2, 4, 8)):
+    """Time every (waves_per_block, waves_per_eu, theta_rows) on an N x M
+    grid; returns (best, {candidate: us per stage launch})."""
     times = {}
-    for wpb, wpe in itertools.product((1, 2, 4), (2, 4)):
+    for wpb, wpe, tr in itertools.product((1, 2, 4), (2, 3, 4), rows):
         p = _probe_params(N, M, integrator, steps * repeats + 10)
         with teuk.Solver(p) as s:
-            s.set_tiling(wpb, wpe)
+            s.set_tiling(wpb, wpe, tr)
             s.step(5)
             s.sync()
             s.enable_timing(True)
@@ -93,11 +98,11 @@ def tune_solver(N, M, integrator=1, steps=60, repeats=2, store=True):
                 s.step(steps)
             s.sync()
             n, ms = s.timing()
-        times[(wpb, wpe)] = 1e3 * ms / n
+        times[(wpb, wpe, tr)] = 1e3 * ms / n
     best = min(times, key=times.get)
     if store:
         _store(solver_key(N, M, integrator), {"waves_per_block": best[0], "waves_per_eu": best[1],
-                                              "us_per_launch": times[best]})
+                                              "theta_rows": best[2], "us_per_launch": times[best]})
     return best, times
 
 
@@ -106,17 +111,18 @@ def tune_solver(N, M, integrator=1, steps=60, repeats=2, store=True):
 GROUP_BYTES = (0, 256 << 20)
 
 
-def tune_batch(shapes, integrator=1, steps=64, store=True, chunks=(16, 32, 64), group_bytes=GROUP_BYTES):
+def tune_batch(shapes, integrator=1, steps=64, store=True, chunks=(16, 32, 64), group_bytes=GROUP_BYTES,
+               rows=(2, 4, 8), wpes=(2, 3)):
     """Time whole batch steps of probe solvers of the given (N, M) shapes
     (one probe per entry, so repeat a shape to tune a batch of many) for
-    every (waves_per_block, waves_per_eu, order, chunk, cache-group bytes);
-    returns (best, {candidate: seconds per step})."""
+    every (waves_per_block, waves_per_eu, order, chunk, cache-group bytes,
+    theta_rows); returns (best, {candidate: seconds per step})."""
     times = {}
-    for wpb, wpe, order, chunk, gb in itertools.product((1, 2, 4), (2, 4), (0, 1), chunks, group_bytes):
+    for wpb, wpe, order, chunk, gb, tr in itertools.product((1, 2, 4), wpes, (0, 1), chunks, group_bytes, rows):
         solvers = [teuk.Solver(_probe_params(n, m, integrator, 3 * steps + 10)) for n, m in shapes]
         try:
             with teuk.Batch(integrator=integrator) as b:
-                b.set_tiling(wpb, wpe, order, chunk)
+                b.set_tiling(wpb, wpe, order, chunk, tr)
                 b.set_cache_groups(gb)
                 for s in solvers:
                     b.add(s)
@@ -125,7 +131,7 @@ def tune_batch(shapes, integrator=1, steps=64, store=True, chunks=(16, 32, 64), 
                 t0 = time.perf_counter()
                 b.step(2 * steps)
                 b.sync()
-                times[(wpb, wpe, order, chunk, gb)] = (time.perf_counter() - t0) / (2 * steps)
+                times[(wpb, wpe, order, chunk, gb, tr)] = (time.perf_counter() - t0) / (2 * steps)
         finally:
             for s in solvers:
                 s.close()
@@ -133,14 +139,18 @@ def tune_batch(shapes, integrator=1, steps=64, store=True, chunks=(16, 32, 64), 
     if store:
         _store(batch_key(shapes, integrator), {"waves_per_block": best[0], "waves_per_eu": best[1],
                                                "order": best[2], "chunk": best[3], "group_bytes": best[4],
-                                               "s_per_step": times[best]})
+                                               "theta_rows": best[5], "s_per_step": times[best]})
     return best, times
 
 
 def best_solver_tiling(N, M, integrator=1):
-    """Tuned (waves_per_block, waves_per_eu) for a grid, or None."""
+    """Tuned launch shape for a grid as Solver.set_tiling keyword arguments,
+    or None."""
     e = _load().get(solver_key(N, M, integrator))
-    return (e["waves_per_block"], e["waves_per_eu"]) if e else None
+    if not e:
+        return None
+    return {"waves_per_block": e["waves_per_block"], "waves_per_eu": e["waves_per_eu"],
+            "theta_rows": e.get("theta_rows", 8)}
 
 
 def best_batch_tiling(shapes, integrator=1):
@@ -162,7 +172,7 @@ def best_batch_tiling(shapes, integrator=1):
     if not e:
         return None
     return {"waves_per_block": e["waves_per_block"], "waves_per_eu": e["waves_per_eu"], "order": e["order"],
-            "chunk": e["chunk"]}
+            "chunk": e["chunk"], "theta_rows": e.get("theta_rows", 8)}
 
 
 def best_batch_groups(shapes, integrator=1):
@@ -179,18 +189,19 @@ def main(argv=None):
     ap.add_argument("--solver", nargs="*", default=[])
     ap.add_argument("--batch", nargs="*", default=[])
     ap.add_argument("--integrator", type=int, default=1)
+    ap.add_argument("--chunks", default="16,32,64", help="steps per captured graph tried for batches")
     args = ap.parse_args(argv)
     for s in args.solver:
         N, M = map(int, s.split("x"))
         best, t = tune_solver(N, M, args.integrator)
-        print(json.dumps({"solver": s, "best": best, "us": {f"{k[0]},{k[1]}": round(v, 3) for k, v in t.items()}}),
+        print(json.dumps({"solver": s, "best": best, "us": {",".join(map(str, k)): round(v, 3) for k, v in t.items()}}),
               flush=True)
     for spec in args.batch:
         shapes = []
         for x in spec.split(","):   # NxM or NxM*count
             sh, _, cnt = x.partition("*")
             shapes += [tuple(map(int, sh.split("x")))] * int(cnt or 1)
-        best, t = tune_batch(shapes, args.integrator)
+        best, t = tune_batch(shapes, args.integrator, chunks=tuple(int(c) for c in args.chunks.split(",")))
         print(json.dumps({"batch": spec, "best": best,
                           "ms_per_step": {",".join(map(str, k)): round(1e3 * v, 4) for k, v in t.items()}}),
               flush=True)

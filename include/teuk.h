@@ -177,11 +177,22 @@ int teuk_timing(teuk_solver *s, long long *launches, double *ms);
 long long teuk_footprint(const teuk_params *p, int nsteps);
 
 /* Launch shape of the stage kernels: wavefronts per block (1, 2, 4; each
- * wavefront takes 8 theta rows, and when the M/8 row chunks are not a
- * multiple of it the surplus wavefronts of a strip's last block stay idle)
- * and the occupancy the kernels were compiled for (2 or 4 waves per SIMD).
- * Results do not depend on it. */
+ * wavefront takes one chunk of theta rows, see teuk_set_theta_rows, and
+ * when the chunks of a rho strip are not a multiple of it the surplus
+ * wavefronts of the strip's last block stay idle) and the occupancy the
+ * kernels were compiled for (2, 3 or 4 waves per SIMD).  Results do not depend
+ * on it. */
 int teuk_set_tiling(teuk_solver *s, int waves_per_block, int waves_per_eu);
+
+/* Theta rows per stage-kernel wavefront: 2 (the default), 4 or 8.  The
+ * first and last chunk of a rho strip compute 2, 3 or 5 rows next to the
+ * theta boundary rows they fill, the chunks between `rows` rows each; a
+ * 64-column strip of an M-row grid then takes 2 + (M - 10) / 2,
+ * 2 + (M - 12) / 4 or M / 8 wavefronts.  Short chunks give more wavefronts
+ * and fewer registers per wavefront, long ones re-read fewer theta halo
+ * rows.  Results do not depend on it.  No counterpart in the reference (its
+ * kernels take one thread per point, body.cu:131-165). */
+int teuk_set_theta_rows(teuk_solver *s, int rows);
 
 /* The 12 per-step source coefficients (the canonical T_4 coefficients times
  * e^{-i m phi_p}/pi, (0,0) (1,0) (0,1) (2,0) (1,1) (0,2) as re, im pairs)
@@ -222,6 +233,9 @@ int teuk_batch_remove(teuk_batch *b, teuk_solver *s);
 /* order 0: each XCD gets a contiguous run of tiles; 1: tiles dealt in list
  * order.  chunk: steps per captured graph. */
 int teuk_batch_set_tiling(teuk_batch *b, int waves_per_block, int waves_per_eu, int order, int chunk);
+/* Theta rows per stage-kernel wavefront for every member (as
+ * teuk_set_theta_rows). */
+int teuk_batch_set_theta_rows(teuk_batch *b, int rows);
 /* Cache groups: the unfinished members, in list order, are split into
  * groups whose field state (RK state planes and the dbdx plane, 136 B per
  * point for RK4) stays within group_bytes, so that a group's fields stay in

@@ -45,18 +45,21 @@ CASES = [  # N, M, xmin, xmax, a, m, steps, seed, source
 
 
 @pytest.mark.parametrize("integrator", [1, 0])
-@pytest.mark.parametrize("wpb,wpe,order,chunk,groups", [(2, 2, 0, 32, 0), (1, 4, 1, 7, 0), (4, 2, 0, 16, 0),
-                                                        (4, 2, 0, 32, 2 << 20), (2, 2, 1, 16, 1)])
-def test_batch_members_equal_alone(integrator, wpb, wpe, order, chunk, groups):
+@pytest.mark.parametrize("wpb,wpe,order,chunk,groups,rows", [(2, 2, 0, 32, 0, 2), (1, 4, 1, 7, 0, 8),
+                                                             (4, 3, 0, 16, 0, 4), (4, 2, 0, 32, 2 << 20, 2),
+                                                             (2, 2, 1, 16, 1, 8)])
+def test_batch_members_equal_alone(integrator, wpb, wpe, order, chunk, groups, rows):
     """groups: cache-group budget in bytes (0 = one group; 1 = a group per
-    member; 2 MiB = two or three members per group here)."""
+    member; 2 MiB = two or three members per group here); rows: theta rows
+    per wavefront (members run alone with the default 2)."""
     specs = [member(N, M, x0, x1, a, m, st, integrator, seed, src) for N, M, x0, x1, a, m, st, seed, src in CASES]
     steps = [c[6] for c in CASES]
-    print(f"batch integ={integrator} wpb={wpb} wpe={wpe} order={order} chunk={chunk} groups={groups}", flush=True)
+    print(f"batch integ={integrator} wpb={wpb} wpe={wpe} order={order} chunk={chunk} groups={groups} "
+          f"rows={rows}", flush=True)
     alone = [run_alone(p, u0, rows, st) for (p, u0, rows), st in zip(specs, steps)]
     solvers = []
     with teuk.Batch(integrator=integrator) as b:
-        b.set_tiling(wpb, wpe, order, chunk)
+        b.set_tiling(wpb, wpe, order, chunk, rows)
         b.set_cache_groups(groups)
         for (p, u0, rows), st in zip(specs, steps):
             _, r_c, th = teuk.grid(p)
@@ -113,15 +116,16 @@ def test_solver_in_batch_refuses_own_stepping():
 @pytest.mark.parametrize("M", [32, 40])
 def test_single_solver_tiling_is_bitwise_neutral(M):
     """Every launch shape gives the same bits, also where the waves per block
-    do not divide the theta chunks (M = 40: 5 chunks)."""
+    do not divide the theta chunks (M = 40: 5 chunks of 8 rows, 9 of 4 rows,
+    17 of 2 rows)."""
     p, u0, rows = member(512, M, -40.0, 60.0, 0.6, 2, 40, 1, 5, True)
     ref = run_alone(p, u0, rows, 40)[0]
     _, r_c, th = teuk.grid(p)
-    for wpb, wpe in [(1, 2), (2, 4), (4, 4)]:
+    for wpb, wpe, rows_ in [(1, 2, 8), (2, 4, 2), (4, 4, 4), (4, 2, 8), (2, 2, 4), (1, 3, 2), (2, 3, 4)]:
         with teuk.Solver(p, r_c, th) as s:
-            s.set_tiling(wpb, wpe)
+            s.set_tiling(wpb, wpe, rows_)
             s.set_fields(u0)
             s.set_trajectory(rows, 40)
             s.step(40)
             s.sync()
-            assert np.array_equal(s.get_fields(), ref), (wpb, wpe)
+            assert np.array_equal(s.get_fields(), ref), (wpb, wpe, rows_)

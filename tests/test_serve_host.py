@@ -61,13 +61,16 @@ def test_tuner_defaults_and_local_cache(tmp_path, monkeypatch):
     defaults = json.load(open(tune.DEFAULTS))
     key = tune.solver_key(16384, 64, 1)
     assert key in defaults
-    want = (defaults[key]["waves_per_block"], defaults[key]["waves_per_eu"])
-    assert tune.best_solver_tiling(16384, 64, 1) == want
+    def shape(e):
+        return {"waves_per_block": e["waves_per_block"], "waves_per_eu": e["waves_per_eu"],
+                "theta_rows": e.get("theta_rows", 8)}
+
+    assert tune.best_solver_tiling(16384, 64, 1) == shape(defaults[key])
     other = tune.solver_key(4096, 32, 1)
-    cache.write_text(json.dumps({key: {"waves_per_block": 1, "waves_per_eu": 4, "us_per_launch": 1.0}}))
-    assert tune.best_solver_tiling(16384, 64, 1) == (1, 4)
-    assert tune.best_solver_tiling(4096, 32, 1) == (defaults[other]["waves_per_block"],
-                                                     defaults[other]["waves_per_eu"])
+    cache.write_text(json.dumps({key: {"waves_per_block": 1, "waves_per_eu": 4, "theta_rows": 4,
+                                       "us_per_launch": 1.0}}))
+    assert tune.best_solver_tiling(16384, 64, 1) == {"waves_per_block": 1, "waves_per_eu": 4, "theta_rows": 4}
+    assert tune.best_solver_tiling(4096, 32, 1) == shape(defaults[other])
     assert tune.best_solver_tiling(1024, 16, 1) is None
 
 

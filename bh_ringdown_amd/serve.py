@@ -192,7 +192,11 @@ class Server:
         if self.tiling:
             batch.set_tiling(chunk=self.chunk, **self.tiling)
         else:
-            batch.set_tiling(4, 2, 0, self.chunk)
+            # measured on an MI355X over the 16-request window of the
+            # Poisson stream at full lengths (N 4096-16384, M 32/64, about
+            # 1 M resident points): 8-row wavefronts, 2 per block, 11.2 s;
+            # 2-row wavefronts 11.5-12.7 s
+            batch.set_tiling(2, 2, 0, self.chunk, theta_rows=8)
         try:
             while pending or queue or live:
                 if deadline is not None and now() >= deadline:

@@ -1042,10 +1042,19 @@ int batch_rebuild(teuk_batch *b) {
   // descriptor).  Blocks are dealt round-robin over the 8 XCDs: block i lands
   // on XCD i % 8.  Order 0 gives XCD x the contiguous run x*per ..
   // (x+1)*per of the group's list, so neighbouring tiles of one solve share
-  // an L2; no-op tiles pad the table.
+  // an L2; no-op tiles pad the table.  Within each XCD's run (order 0) or
+  // the whole list (order 1) the block-rows holding a strip's first or last
+  // theta chunk come first: their waves also fill the theta boundary rows,
+  // the longest of the launch, and dispatched last they would set its end.
   BatchTile pad;
   std::memset(&pad, 0, sizeof pad);
   pad.tile = -1;
+  auto edge_first = [](std::vector<BatchTile>::iterator first, std::vector<BatchTile>::iterator last) {
+    std::stable_partition(first, last, [](const BatchTile &bt) {
+      const int grp = bt.tile % bt.sd.ngroups;
+      return grp == 0 || grp == bt.sd.ngroups - 1;
+    });
+  };
   for (auto &g : b->groups) {
     std::vector<BatchTile> list;
     for (int i = 0; i < g.count; i++)
@@ -1058,11 +1067,16 @@ int batch_rebuild(teuk_batch *b) {
     g.tile_off = table.size();
     if (b->order == 0) {
       const size_t T = list.size(), per = (T + 7) / 8;
+      for (size_t x = 0; x < 8; x++) {
+        const size_t lo = std::min(T, x * per), hi = std::min(T, (x + 1) * per);
+        edge_first(list.begin() + lo, list.begin() + hi);
+      }
       for (size_t blk = 0; blk < per * 8; blk++) {
         const size_t src = (blk & 7) * per + (blk >> 3);
         table.push_back(src < T ? list[src] : pad);
       }
     } else {
+      edge_first(list.begin(), list.end());
       table.insert(table.end(), list.begin(), list.end());
     }
     g.ntiles = (int)(table.size() - g.tile_off);

@@ -194,9 +194,10 @@ class Server:
         else:
             # measured on an MI355X over the 16-request window of the
             # Poisson stream at full lengths (N 4096-16384, M 32/64, about
-            # 1 M resident points): 8-row wavefronts, 2 per block, 11.2 s;
-            # 2-row wavefronts 11.5-12.7 s
-            batch.set_tiling(2, 2, 0, self.chunk, theta_rows=8)
+            # 1 M resident points): 2-row wavefronts, 1 per block, built for
+            # 3 waves per SIMD, 11.07 s; 4-row 11.21 s; 8-row 2 per block
+            # 11.35 s
+            batch.set_tiling(1, 3, 0, self.chunk, theta_rows=2)
         try:
             while pending or queue or live:
                 if deadline is not None and now() >= deadline:

@@ -1042,14 +1042,18 @@ int batch_rebuild(teuk_batch *b) {
   // descriptor).  Blocks are dealt round-robin over the 8 XCDs: block i lands
   // on XCD i % 8.  Order 0 gives XCD x the contiguous run x*per ..
   // (x+1)*per of the group's list, so neighbouring tiles of one solve share
-  // an L2; no-op tiles pad the table.  Within each XCD's run (order 0) or
-  // the whole list (order 1) the block-rows holding a strip's first or last
-  // theta chunk come first: their waves also fill the theta boundary rows,
-  // the longest of the launch, and dispatched last they would set its end.
+  // an L2; no-op tiles pad the table.  With 2-row theta chunks, within each
+  // XCD's run (order 0) or the whole list (order 1) the block-rows holding a
+  // strip's first or last chunk come first: their waves also fill the theta
+  // boundary rows, the longest of the launch, and dispatched last they would
+  // set its end (4- and 8-row edge chunks compute fewer rows than the others,
+  // and there the plain order measured faster).
   BatchTile pad;
   std::memset(&pad, 0, sizeof pad);
   pad.tile = -1;
-  auto edge_first = [](std::vector<BatchTile>::iterator first, std::vector<BatchTile>::iterator last) {
+  const bool edge_rows_first = b->tiling.rows == 2;
+  auto edge_first = [edge_rows_first](std::vector<BatchTile>::iterator first, std::vector<BatchTile>::iterator last) {
+    if (!edge_rows_first) return;
     std::stable_partition(first, last, [](const BatchTile &bt) {
       const int grp = bt.tile % bt.sd.ngroups;
       return grp == 0 || grp == bt.sd.ngroups - 1;

@@ -165,6 +165,50 @@ TEUK_HD constexpr bool theta_rows_ok(int M, int R) {
 static_assert(theta_chunks(16, 8) == 2 && theta_chunks(64, 8) == 8 && theta_chunks(64, 4) == 15 &&
                   theta_chunks(64, 2) == 29 && theta_chunks(16, 2) == 5,
               "theta chunk layout");
+// First row of theta chunk c, 0 <= c < theta_chunks(M, R): the first chunk
+// starts at row 0, the last at theta_last_chunk_b0, chunk c in between at
+// theta_inner_chunk_b0.  The edge chunks span edge_rows(R) + 3 rows (their
+// computed rows and the three boundary rows), the others R.  (The stage
+// kernel calls the per-kind forms inside its per-kind branches: a first row
+// the compiler cannot see as 0 in the first chunk cost 15 % of the stage.)
+TEUK_HD constexpr int theta_last_chunk_b0(int M, int R) { return M - edge_rows(R) - 3; }
+TEUK_HD constexpr int theta_inner_chunk_b0(int R, int c) { return edge_rows(R) + 3 + (c - 1) * R; }
+TEUK_HD constexpr int theta_chunk_b0(int M, int R, int c) {
+  return c == 0 ? 0 : (c == theta_chunks(M, R) - 1 ? theta_last_chunk_b0(M, R) : theta_inner_chunk_b0(R, c));
+}
+
+// Block b of a single-solve stage launch of nblocks = S * ngroups blocks (S
+// rho strips, ngroups block-rows per strip): its strip and block-row.  The
+// hardware deals blocks round-robin over the 8 XCDs in dispatch order; each
+// XCD gets a contiguous run of tiles, so the halos its waves re-read stay in
+// its own L2.  edge_first: XCD x owns strips [x S/8, (x+1) S/8) when the
+// strips split evenly, and the block-rows holding a strip's first and last
+// theta chunk are dispatched before the others.
+TEUK_HD void stage_block_tile(int b, int nblocks, int ngroups, bool edge_first, int &strip, int &grp) {
+  if (!edge_first) {
+    int tile = b;
+    if ((nblocks & 7) == 0) tile = (b & 7) * (nblocks >> 3) + (b >> 3);
+    strip = tile / ngroups;
+    grp = tile - strip * ngroups;
+    return;
+  }
+  const int S = nblocks / ngroups;
+  int j = b, s0 = 0, ns = S;
+  if ((S & 7) == 0) {
+    ns = S >> 3;
+    s0 = (b & 7) * ns;
+    j = b >> 3;
+  }
+  const int ne = ngroups >= 2 ? 2 : 1;   // edge block-rows per strip
+  if (j < ne * ns) {
+    strip = s0 + j / ne;
+    grp = (j % ne) ? ngroups - 1 : 0;
+  } else {
+    const int jj = j - ne * ns;
+    strip = s0 + jj / (ngroups - 2);
+    grp = 1 + jj % (ngroups - 2);
+  }
+}
 constexpr int kDefaultThetaRows = 2;
 constexpr int kDefaultWavesPerEU = 2;   // a batch's until set (teuk_batch_set_tiling)
 

@@ -2,8 +2,9 @@
 // UndefinedBehaviorSanitizer (tests/test_host_sanitizers.py builds it with
 // g++ -fsanitize=address,undefined and runs it): the grid inversions, the
 // coefficient and projection tables, the source preparation and T4
-// coefficients, and the trajectory generator over steep, fast, near-polar
-// and high-spin worldlines.  Every output must be finite; any out-of-bounds
+// coefficients, the trajectory generator over steep, fast, near-polar and
+// high-spin worldlines, and the stage kernels' theta-chunk layout and block
+// order.  Every output must be finite; any out-of-bounds
 // access or undefined operation aborts the run.
 #include <cmath>
 #include <cstdio>
@@ -91,9 +92,67 @@ void trajectory_and_source(double a, double eta, double iota_deg, double dt_out,
   }
 }
 
+// The stage kernels' theta chunks (theta_chunks, theta_chunk_b0) cover the
+// rows 0 .. M-1 once and compute the rows 3 .. M-4 once, each edge chunk
+// holding the two rows it extrapolates from, for every M the solver accepts;
+// the single-solve block order (stage_block_tile) visits every (strip,
+// block-row) once, in both orders.
+void chunk_layout_and_block_order() {
+  using namespace teuk;
+  for (int M = 16; M <= 1024; M += 8)
+    for (int R : {2, 4, 8}) {
+      const int nc = theta_chunks(M, R), E = edge_rows(R);
+      std::vector<int> spanned(M, 0), computed(M, 0);
+      for (int c = 0; c < nc; c++) {
+        const bool first = c == 0, last = c == nc - 1;
+        const int b0 = theta_chunk_b0(M, R, c), span = (first || last) ? E + 3 : R;
+        const int c0 = b0 + (first ? 3 : 0), c1 = b0 + (last ? span - 3 : span);   // computed rows [c0, c1)
+        if (b0 < 0 || b0 + span > M) {
+          std::printf("chunk %d of M=%d R=%d outside the grid\n", c, M, R);
+          failures++;
+          continue;
+        }
+        for (int r = b0; r < b0 + span; r++) spanned[r]++;
+        for (int r = c0; r < c1; r++) computed[r]++;
+        if ((first && !(c0 <= 3 && 4 < c1)) || (last && !(c0 <= M - 5 && M - 4 < c1))) {
+          std::printf("edge chunk %d of M=%d R=%d lacks its extrapolation rows\n", c, M, R);
+          failures++;
+        }
+      }
+      for (int r = 0; r < M; r++)
+        if (spanned[r] != 1 || computed[r] != (r >= 3 && r < M - 3 ? 1 : 0)) {
+          std::printf("row %d of M=%d R=%d: spanned %d, computed %d times\n", r, M, R, spanned[r], computed[r]);
+          failures++;
+          break;
+        }
+    }
+  for (int S : {1, 2, 3, 5, 8, 12, 16, 24, 40, 64, 256})
+    for (int ng = 1; ng <= 40; ng++)
+      for (bool edge_first : {false, true}) {
+        std::vector<int> seen((size_t)S * ng, 0);
+        for (int b = 0; b < S * ng; b++) {
+          int strip = -1, grp = -1;
+          stage_block_tile(b, S * ng, ng, edge_first, strip, grp);
+          if (strip < 0 || strip >= S || grp < 0 || grp >= ng) {
+            std::printf("block %d of %dx%d maps outside\n", b, S, ng);
+            failures++;
+            return;
+          }
+          seen[(size_t)strip * ng + grp]++;
+        }
+        for (int v : seen)
+          if (v != 1) {
+            std::printf("block order %dx%d (edge first %d) is not a permutation\n", S, ng, (int)edge_first);
+            failures++;
+            break;
+          }
+      }
+}
+
 }  // namespace
 
 int main() {
+  chunk_layout_and_block_order();
   grids_and_tables(1024, 32, -100.0, 300.0, 0.5, 2.0);
   grids_and_tables(512, 16, -40.0, 60.0, 0.99, -3.0);
   grids_and_tables(256, 64, -50.0, 50.0, 1e-8, 0.0);

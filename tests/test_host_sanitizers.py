@@ -1,8 +1,10 @@
 """libteuk's host code under AddressSanitizer and UndefinedBehaviorSanitizer
 (no GPU): tests/sanitize/host_checks.cpp drives the grid inversions, the
 coefficient, projection and phase tables, the per-step source preparation
-and T4 coefficients, and the trajectory generator over steep, fast, near-
-polar and high-spin worldlines, built with g++ -fsanitize=address,undefined
+and T4 coefficients, the trajectory generator over steep, fast, near-polar
+and high-spin worldlines, and the stage kernels' theta-chunk layout and
+block order (every row spanned and computed once, every tile visited once),
+built with g++ -fsanitize=address,undefined
 from the same sources as the library.  Any out-of-bounds access, use after
 free, signed overflow or other undefined operation aborts the run, and every
 output must be finite.  (The reference has no such checks; its host code

@@ -54,9 +54,32 @@ def test_theta_rows_bitwise_against_oracle(theta_rows, wpb, N, M, integrator):
     assert np.allclose(out["h"], ho, rtol=1e-8, atol=1e-8 * np.abs(ho).max())
 
 
-def test_theta_rows_rejected_outside_2_4_8():
+def test_launch_shapes_rejected_outside_their_ranges():
+    """Theta rows other than 2, 4, 8 and waves per SIMD other than 2-4 are
+    refused, for solvers and batches; the staggered scheme has no stage
+    kernel and refuses both setters."""
     p = teuk.default_params(256, 16, -40.0, 60.0, 0.7, 2, dt_over_dx=0.5)
+    p.T = 5 * p.dt * (1 + 1e-12)
     with teuk.Solver(p) as s:
         for bad in (0, 1, 3, 16):
             with pytest.raises(teuk.TeukError):
                 s.set_tiling(1, 2, bad)
+        for wpe in (1, 5):
+            with pytest.raises(teuk.TeukError):
+                s.set_tiling(1, wpe)
+        s.set_tiling(1, 3, 4)      # in range: accepted
+        s.step(3)
+        s.sync()
+    with teuk.Batch(integrator=1) as b:
+        for bad in (0, 3, 6):
+            with pytest.raises(teuk.TeukError):
+                b.set_tiling(1, 2, 0, 32, bad)
+        with pytest.raises(teuk.TeukError):
+            b.set_tiling(1, 5, 0, 32)
+    q = teuk.default_params(256, 16, -40.0, 60.0, 0.7, 2, dt_over_dx=0.5)
+    q.integrator = 2
+    with teuk.Solver(q) as s:
+        with pytest.raises(teuk.TeukError):
+            s.set_tiling(1, 2)
+        with pytest.raises(teuk.TeukError):
+            s.set_tiling(1, 2, 2)

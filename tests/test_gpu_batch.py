@@ -45,13 +45,15 @@ CASES = [  # N, M, xmin, xmax, a, m, steps, seed, source
 
 
 @pytest.mark.parametrize("integrator", [1, 0])
-@pytest.mark.parametrize("wpb,wpe,order,chunk,groups,rows", [(2, 2, 0, 32, 0, 2), (1, 4, 1, 7, 0, 8),
+@pytest.mark.parametrize("wpb,wpe,order,chunk,groups,rows", [(2, 2, 0, 32, 0, 2), (1, 4, 1, 7, 0, 2),
                                                              (4, 3, 0, 16, 0, 4), (4, 2, 0, 32, 2 << 20, 2),
                                                              (2, 2, 1, 16, 1, 8)])
 def test_batch_members_equal_alone(integrator, wpb, wpe, order, chunk, groups, rows):
     """groups: cache-group budget in bytes (0 = one group; 1 = a group per
     member; 2 MiB = two or three members per group here); rows: theta rows
-    per wavefront (members run alone with the default 2)."""
+    per wavefront (members run alone with the default 2; with 2-row
+    wavefronts the tile table puts the edge chunks first, within each XCD's
+    run for order 0 and over the whole list for order 1)."""
     specs = [member(N, M, x0, x1, a, m, st, integrator, seed, src) for N, M, x0, x1, a, m, st, seed, src in CASES]
     steps = [c[6] for c in CASES]
     print(f"batch integ={integrator} wpb={wpb} wpe={wpe} order={order} chunk={chunk} groups={groups} "
